@@ -1,0 +1,224 @@
+"""Benchmark: SPHINCS+-SHA2-128f-simple batched signing and verification on
+one MI355X (inputs resident in HBM), with the CPU oracle timed on the host
+cores beside it.
+
+Workload (one "step"): one batched sign launch over B distinct keypairs
+(default 4096) with one 32-byte message each, then one batched verify launch
+over the B signatures.  Keys follow the NIST KAT protocol in DRBG mode:
+master DRBG entropy 00..2f, count i draws seed_i (48 B) and msg_i (32 B);
+DRBG(seed_i) then yields the 3n-byte key seed and the n-byte optrand.
+
+Prints one JSON line.  `value` is signs/s (B / mean batched-sign time);
+verifies/s is reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12   # MI355X: 256 CUs x 64 lanes x 2.4 GHz
+SHA256_OPS = 64 * 14 + 48 * 10 + 8        # VALU ops of one generic compression on gfx950
+KECCAK_OPS = 24 * 182                      # VALU ops of one Keccak-f[1600] on 32-bit halves
+
+
+def kernel_work(p, batch):
+    """Compressions (SHA-2) or permutations (SHAKE) of one k_ht sign launch."""
+    th = p.full_h // p.d
+    leaves = batch * p.d * (1 << th)
+    nodes = batch * p.d * ((1 << th) - 1)
+    if p.hash == 0:
+        tlen = -(-(22 + p.len * p.n + 9) // 64)
+        hblk = -(-(22 + 2 * p.n + 9) // 64)
+        return leaves * (p.len * 16 + tlen) + nodes * hblk
+    tlen = (p.n + 32 + p.len * p.n) // 136 + 1
+    return leaves * (p.len * 16 + tlen) + nodes
+
+
+def cpu_baseline(pid, seconds, procs):
+    """Time the oracle (single-threaded C port of the reference host path) in
+    `procs` forked processes; returns aggregate signs/s and verifies/s."""
+    import multiprocessing as mp
+
+    def work(q, k):
+        from oracle import oracle as O
+        import random
+        p = O.params(pid)
+        rng = random.Random(k)
+        pk, sk = O.seed_keypair(pid, rng.randbytes(3 * p.n))
+        sigs, n_sign, t0 = [], 0, time.perf_counter()
+        while time.perf_counter() - t0 < 0.8 * seconds or n_sign == 0:
+            m = rng.randbytes(32)
+            sigs.append((m, O.signature_with_optrand(pid, m, sk, rng.randbytes(p.n))))
+            n_sign += 1
+        t_sign = time.perf_counter() - t0
+        n_ver, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 0.2 * seconds or n_ver == 0:
+            m, s = sigs[n_ver % len(sigs)]
+            assert O.verify(pid, s, m, pk) == 0
+            n_ver += 1
+        q.put((n_sign / t_sign, n_ver / (time.perf_counter() - t0)))
+
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    ps = [ctx.Process(target=work, args=(q, k)) for k in range(procs)]
+    for p in ps:
+        p.start()
+    res = [q.get() for _ in ps]
+    for p in ps:
+        p.join()
+    return sum(r[0] for r in res), sum(r[1] for r in res), res[0][0]
+
+
+def make_inputs(C, pid, batch):
+    """KAT-protocol inputs: per-key 3n-byte seeds, optrand, 32-byte messages."""
+    import numpy as np
+    p = C.params(pid)
+    C.randombytes_init(bytes(range(48)))
+    seeds48, msgs = [], []
+    for _ in range(batch):
+        seeds48.append(C.randombytes(48))
+        msgs.append(C.randombytes(32))
+    kseeds, opts = [], []
+    for s in seeds48:
+        C.randombytes_init(s)
+        kseeds.append(C.randombytes(3 * p.n))   # crypto_sign_keypair draw (sign.cu:345)
+        opts.append(C.randombytes(p.n))         # crypto_sign optrand draw (sign.cu:1950)
+    to = lambda xs: np.frombuffer(b"".join(xs), np.uint8).copy()
+    return to(kseeds), to(opts), to(msgs)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--param", default="sha2-128f")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-procs", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X "
+              "(signatures shard by request; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cuspx_amd as C
+
+    pid = C.param_id(args.param)
+    p = C.params(pid)
+    B = args.batch
+
+    cpu = None
+    if not args.no_cpu:  # before the GPU is touched: the baseline forks
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        print("cpu baseline: %d processes, %.0f s" % (procs, args.cpu_seconds), file=sys.stderr,
+              flush=True)
+        cs, cv, c1 = cpu_baseline(pid, args.cpu_seconds, procs)
+        cpu = {"value": round(cs, 3), "unit": "signs/s", "cores": procs, "kind": "port",
+               "verifies_per_s": round(cv, 2), "signs_per_s_1core": round(c1, 3),
+               "sample": "oracle/spx_oracle.c %s sign (32-byte messages) for %.0f s then verify "
+                         "for %.0f s in each of %d processes" % (args.param, 0.8 * args.cpu_seconds,
+                                                                 0.2 * args.cpu_seconds, procs)}
+
+    import torch
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    print("inputs: %d keys (KAT protocol, DRBG mode)" % B, file=sys.stderr, flush=True)
+    kseeds, opts, msgs = make_inputs(C, pid, B)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    d_seed, d_opt, d_msg = t(kseeds), t(opts), t(msgs)
+    d_moff = t(np.arange(B, dtype=np.uint64) * 32)
+    d_mlen = t(np.full(B, 32, np.uint32))
+    d_pk = torch.empty(B * p.pk_bytes, dtype=torch.uint8, device=dev)
+    d_sk = torch.empty(B * p.sk_bytes, dtype=torch.uint8, device=dev)
+    d_sig = torch.empty(B * p.sig_bytes, dtype=torch.uint8, device=dev)
+    d_st = torch.full((B,), -7, dtype=torch.int32, device=dev)
+    ws = torch.empty(max(C.workspace_bytes(pid, B, op) for op in (0, 1, 2)), dtype=torch.uint8,
+                     device=dev)
+    ptr = lambda x: x.data_ptr()
+    C.keygen_device(pid, B, ptr(d_seed), 3 * p.n, ptr(d_pk), ptr(d_sk), ptr(ws), sh)
+
+    def sign():
+        C.sign_device(pid, B, ptr(d_sk), p.sk_bytes, ptr(d_msg), ptr(d_moff), ptr(d_mlen),
+                      ptr(d_opt), ptr(d_sig), p.sig_bytes, ptr(ws), sh)
+
+    def verify():
+        C.verify_device(pid, B, ptr(d_pk), p.pk_bytes, ptr(d_sig), p.sig_bytes, ptr(d_msg),
+                        ptr(d_moff), ptr(d_mlen), ptr(d_st), ptr(ws), sh)
+
+    print("warmup: %d steps" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        sign()
+        verify()
+    torch.cuda.synchronize()
+    assert int((d_st != 0).sum()) == 0, "batched verify rejected a batched signature"
+
+    C.profile_reset()
+    C.profile_enable(True)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        sign()
+        ev[k][1].record(stream)
+        verify()
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    C.profile_enable(False)
+    prof = C.profile_read()
+    sign_ms = [e[0].elapsed_time(e[1]) for e in ev]
+    ver_ms = [e[1].elapsed_time(e[2]) for e in ev]
+    assert int((d_st != 0).sum()) == 0, "batched verify rejected a batched signature"
+    ms_step = wall * 1e3 / args.steps
+    sign_s = sum(sign_ms) / len(sign_ms) / 1e3
+    ver_s = sum(ver_ms) / len(ver_ms) / 1e3
+
+    kt, kn = prof.get("k_ht", (0.0, 0))
+    k_avg = kt / max(kn, 1) / 1e3
+    work = kernel_work(p, B)
+    ops = work * (SHA256_OPS if p.hash == 0 else KECCAK_OPS)
+    achieved = ops / k_avg / 1e12 if k_avg > 0 else 0.0
+    out = {
+        "metric": "SHA2-128f signs/sec + verifies/sec on 1 GPU; p50/p99 sign latency at fixed QPS",
+        "value": round(B / sign_s, 1),
+        "unit": "signs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "SPHINCS+-%s-simple batched sign then batched verify, %d distinct "
+                               "DRBG keypairs (KAT counts 0..%d), one 32-byte message each"
+                               % (p.name.decode(), B, B - 1),
+                   "param": p.name.decode(), "batch": B},
+        "verifies_per_s": round(B / ver_s, 1),
+        "sign_batch_ms": round(sign_s * 1e3, 3),
+        "verify_batch_ms": round(ver_s * 1e3, 3),
+        "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
+        "roofline": {"bound": "valu", "kernel": "k_ht", "achieved": round(achieved, 3),
+                     "peak": round(PEAK_INT_TOPS, 2), "unit": "TOP/s",
+                     "frac": round(achieved / PEAK_INT_TOPS, 4), "traffic": None,
+                     "work_per_launch": work,
+                     "work_unit": "sha256 compressions" if p.hash == 0 else "keccak-f permutations",
+                     "ops_per_unit": SHA256_OPS if p.hash == 0 else KECCAK_OPS,
+                     "kernel_ms": round(k_avg * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

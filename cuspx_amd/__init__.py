@@ -1,0 +1,326 @@
+"""cuspx_amd -- MI355X SPHINCS+ engine (host-side binding of libcuspx.so).
+
+The compute path is the HIP library ``cuspx_amd/libcuspx.so`` (C ABI in
+``include/cuspx.h``).  This module is a thin ctypes mirror of the
+reference's interface:
+
+* the api.h surface -- ``crypto_sign_keypair``, ``crypto_sign``,
+  ``crypto_sign_open``, ... (api.h:19-183 of the reference) acting on the
+  default parameter set;
+* the batched data-parallel entry points ``face_mdp_crypto_sign_keypair`` /
+  ``face_mdp_crypto_sign`` / ``face_mdp_crypto_sign_open``
+  (sign.cu:1208, 5098, 6856);
+* runtime-parameterised batches with per-item status (``batch_keygen``,
+  ``batch_sign``, ``batch_verify``);
+* device-resident entry points taking raw device pointers (``sign_device``
+  ...), used by bench.py with buffers already in HBM.
+
+Importing raises if the library has not been built: there is no CPU
+fallback for any part of the engine.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuspx.so")
+
+OK, ERR_VERIFY, ERR_PARAM, ERR_HIP, ERR_ARG, ERR_BUDGET = 0, -1, -2, -3, -4, -5
+OP_SIGN, OP_VERIFY, OP_KEYGEN = 0, 1, 2
+
+PARAM_NAMES = [
+    "sha2-128s", "sha2-128f", "sha2-192s", "sha2-192f", "sha2-256s", "sha2-256f",
+    "shake256-128s", "shake256-128f", "shake256-192s", "shake256-192f",
+    "shake256-256s", "shake256-256f",
+]
+
+
+class CuspxError(RuntimeError):
+    pass
+
+
+class ParamInfo(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int) for f in (
+        "pid", "hash", "n", "full_h", "d", "a", "k", "len", "sig_bytes", "pk_bytes",
+        "sk_bytes", "seed_bytes")] + [("name", ctypes.c_char * 32)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libcuspx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libcuspx.so is not built: run `python -m cuspx_amd.build` "
+                              "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u8p, sz = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t
+        L.cuspx_last_error.restype = ctypes.c_char_p
+        L.cuspx_workspace_bytes.restype = ctypes.c_size_t
+        L.cuspx_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_int]
+        L.crypto_sign_secretkeybytes.restype = ctypes.c_ulonglong
+        L.crypto_sign_publickeybytes.restype = ctypes.c_ulonglong
+        L.crypto_sign_bytes.restype = ctypes.c_ulonglong
+        L.crypto_sign_seedbytes.restype = ctypes.c_ulonglong
+        L.randombytes.argtypes = [vp, ctypes.c_ulonglong]
+        L.randombytes_init.argtypes = [vp, vp, ctypes.c_int]
+        L.crypto_sign_seed_keypair.argtypes = [vp, vp, vp]
+        L.crypto_sign_keypair.argtypes = [vp, vp]
+        L.crypto_sign_signature.argtypes = [vp, ctypes.POINTER(sz), vp, sz, vp]
+        L.crypto_sign_verify.argtypes = [vp, sz, vp, sz, vp]
+        L.crypto_sign.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), vp,
+                                  ctypes.c_ulonglong, vp]
+        L.crypto_sign_open.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), vp,
+                                       ctypes.c_ulonglong, vp]
+        L.face_mdp_crypto_sign_keypair.argtypes = [vp, vp, ctypes.c_uint]
+        L.face_mdp_crypto_sign.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), vp,
+                                           ctypes.c_ulonglong, vp, ctypes.c_uint]
+        L.face_mdp_crypto_sign_open.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), vp,
+                                                ctypes.c_ulonglong, vp, ctypes.c_uint]
+        L.spx_batch_keygen.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, vp, vp]
+        L.spx_batch_sign.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
+        L.spx_batch_verify.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
+        L.cuspx_sign_device.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
+                                        vp, vp, vp, vp, ctypes.c_uint64, vp, vp]
+        L.cuspx_verify_device.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
+                                          ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
+        L.cuspx_keygen_device.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
+                                          vp, vp, vp]
+        L.cuspx_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc not in (OK, ERR_VERIFY):
+        raise CuspxError("%s failed (%d): %s" % (what, rc, lib().cuspx_last_error().decode()))
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ------------------------------------------------------------ parameters
+
+def param_id(name):
+    pid = lib().cuspx_param_id(name.encode())
+    if pid < 0:
+        raise KeyError(name)
+    return pid
+
+
+def params(pid):
+    p = ParamInfo()
+    if lib().cuspx_get_param_info(pid, ctypes.byref(p)) != 0:
+        raise KeyError(pid)
+    return p
+
+
+def set_default_param(pid):
+    if lib().cuspx_set_default_param(pid) != 0:
+        raise KeyError(pid)
+
+
+def set_debug_mode(on):
+    lib().cuspx_set_debug_mode(1 if on else 0)
+
+
+def get_debug_mode():
+    return bool(lib().cuspx_get_debug_mode())
+
+
+# ------------------------------------------------------- api.h surface
+
+def randombytes_init(entropy, personalization=None):
+    e = ctypes.create_string_buffer(bytes(entropy), 48)
+    p = ctypes.create_string_buffer(bytes(personalization), 48) if personalization else None
+    lib().randombytes_init(e, p, 256)
+
+
+def randombytes(n):
+    b = ctypes.create_string_buffer(max(n, 1))
+    lib().randombytes(b, n)
+    return b.raw[:n]
+
+
+def crypto_sign_seed_keypair(seed):
+    L = lib()
+    pk = ctypes.create_string_buffer(L.crypto_sign_publickeybytes())
+    sk = ctypes.create_string_buffer(L.crypto_sign_secretkeybytes())
+    _check(L.crypto_sign_seed_keypair(pk, sk, ctypes.create_string_buffer(bytes(seed))),
+           "crypto_sign_seed_keypair")
+    return pk.raw, sk.raw
+
+
+def crypto_sign_keypair():
+    L = lib()
+    pk = ctypes.create_string_buffer(L.crypto_sign_publickeybytes())
+    sk = ctypes.create_string_buffer(L.crypto_sign_secretkeybytes())
+    _check(L.crypto_sign_keypair(pk, sk), "crypto_sign_keypair")
+    return pk.raw, sk.raw
+
+
+def crypto_sign(m, sk):
+    L = lib()
+    sm = ctypes.create_string_buffer(L.crypto_sign_bytes() + len(m))
+    smlen = ctypes.c_ulonglong()
+    _check(L.crypto_sign(sm, ctypes.byref(smlen), ctypes.create_string_buffer(bytes(m), max(len(m), 1)),
+                         len(m), ctypes.create_string_buffer(bytes(sk))), "crypto_sign")
+    return sm.raw[:smlen.value]
+
+
+def crypto_sign_open(sm, pk):
+    """Returns (rc, message); rc is 0 or -1 as in the reference."""
+    L = lib()
+    m = ctypes.create_string_buffer(max(len(sm), 1))
+    mlen = ctypes.c_ulonglong()
+    rc = _check(L.crypto_sign_open(m, ctypes.byref(mlen), ctypes.create_string_buffer(bytes(sm)),
+                                   len(sm), ctypes.create_string_buffer(bytes(pk))),
+                "crypto_sign_open")
+    return rc, m.raw[:mlen.value]
+
+
+def crypto_sign_signature(m, sk):
+    L = lib()
+    sig = ctypes.create_string_buffer(L.crypto_sign_bytes())
+    siglen = ctypes.c_size_t()
+    _check(L.crypto_sign_signature(sig, ctypes.byref(siglen),
+                                   ctypes.create_string_buffer(bytes(m), max(len(m), 1)), len(m),
+                                   ctypes.create_string_buffer(bytes(sk))), "crypto_sign_signature")
+    return sig.raw[:siglen.value]
+
+
+def crypto_sign_verify(sig, m, pk):
+    return _check(lib().crypto_sign_verify(ctypes.create_string_buffer(bytes(sig)), len(sig),
+                                           ctypes.create_string_buffer(bytes(m), max(len(m), 1)),
+                                           len(m), ctypes.create_string_buffer(bytes(pk))),
+                  "crypto_sign_verify")
+
+
+# ------------------------------------------------- face_mdp (sign.cu)
+
+def face_mdp_crypto_sign_keypair(num):
+    L = lib()
+    pk = np.zeros((num, L.crypto_sign_publickeybytes()), np.uint8)
+    sk = np.zeros((num, L.crypto_sign_secretkeybytes()), np.uint8)
+    _check(L.face_mdp_crypto_sign_keypair(_ptr(pk), _ptr(sk), num), "face_mdp_crypto_sign_keypair")
+    return pk, sk
+
+
+def face_mdp_crypto_sign(m, mlen, sk, num):
+    """m: uint8 array of num*32 bytes (SPX_MLEN stride); returns (sm, smlen)."""
+    L = lib()
+    stride = L.crypto_sign_bytes() + 32
+    sm = np.zeros((num, stride), np.uint8)
+    smlen = ctypes.c_ulonglong()
+    m = np.ascontiguousarray(m, np.uint8)
+    sk = np.ascontiguousarray(sk, np.uint8)
+    _check(L.face_mdp_crypto_sign(_ptr(sm), ctypes.byref(smlen), _ptr(m), mlen, _ptr(sk), num),
+           "face_mdp_crypto_sign")
+    return sm, smlen.value
+
+
+def face_mdp_crypto_sign_open(sm, smlen, pk, num):
+    """Returns (rc, m[num][32], mlen)."""
+    L = lib()
+    m = np.zeros((num, 32), np.uint8)
+    mlen = ctypes.c_ulonglong()
+    sm = np.ascontiguousarray(sm, np.uint8)
+    pk = np.ascontiguousarray(pk, np.uint8)
+    rc = _check(L.face_mdp_crypto_sign_open(_ptr(m), ctypes.byref(mlen), _ptr(sm), smlen, _ptr(pk),
+                                            num), "face_mdp_crypto_sign_open")
+    return rc, m, mlen.value
+
+
+# ------------------------------------- runtime-parameterised batches
+
+def _pack(msgs):
+    mlen = np.array([len(m) for m in msgs], np.uint32)
+    moff = np.zeros(len(msgs), np.uint64)
+    if len(msgs):
+        moff[1:] = np.cumsum(mlen[:-1], dtype=np.uint64)
+    buf = np.frombuffer(b"".join(bytes(m) for m in msgs) + b"\0", np.uint8).copy()
+    return buf, moff, mlen
+
+
+def batch_keygen(pid, seeds):
+    p = params(pid)
+    seeds = np.ascontiguousarray(seeds, np.uint8).reshape(-1, 3 * p.n)
+    num = seeds.shape[0]
+    pk = np.zeros((num, p.pk_bytes), np.uint8)
+    sk = np.zeros((num, p.sk_bytes), np.uint8)
+    _check(lib().spx_batch_keygen(pid, num, _ptr(seeds), _ptr(pk), _ptr(sk)), "spx_batch_keygen")
+    return pk, sk
+
+
+def batch_sign(pid, sks, msgs, optrand=None):
+    p = params(pid)
+    sks = np.ascontiguousarray(sks, np.uint8).reshape(-1, p.sk_bytes)
+    num = sks.shape[0]
+    buf, moff, mlen = _pack(msgs)
+    opt = None if optrand is None else np.ascontiguousarray(optrand, np.uint8).reshape(num, p.n)
+    sig = np.zeros((num, p.sig_bytes), np.uint8)
+    _check(lib().spx_batch_sign(pid, num, _ptr(sks), _ptr(buf), _ptr(moff), _ptr(mlen), _ptr(opt),
+                                _ptr(sig)), "spx_batch_sign")
+    return sig
+
+
+def batch_verify(pid, pks, sigs, msgs):
+    p = params(pid)
+    pks = np.ascontiguousarray(pks, np.uint8).reshape(-1, p.pk_bytes)
+    sigs = np.ascontiguousarray(sigs, np.uint8).reshape(-1, p.sig_bytes)
+    num = pks.shape[0]
+    buf, moff, mlen = _pack(msgs)
+    st = np.full(num, -7, np.int32)
+    _check(lib().spx_batch_verify(pid, num, _ptr(pks), _ptr(sigs), _ptr(buf), _ptr(moff),
+                                  _ptr(mlen), _ptr(st)), "spx_batch_verify")
+    return st
+
+
+# --------------------------------------------- device-resident batches
+
+def workspace_bytes(pid, num, op):
+    return lib().cuspx_workspace_bytes(pid, num, op)
+
+
+def sign_device(pid, num, sk, sk_stride, msg, moff, mlen, optrand, sig, sig_stride, ws,
+                stream=None):
+    """All buffers are raw device pointers (ints); asynchronous on stream."""
+    return _check(lib().cuspx_sign_device(pid, num, sk, sk_stride, msg, moff, mlen, optrand, sig,
+                                          sig_stride, ws, stream), "cuspx_sign_device")
+
+
+def verify_device(pid, num, pk, pk_stride, sig, sig_stride, msg, moff, mlen, status, ws,
+                  stream=None):
+    return _check(lib().cuspx_verify_device(pid, num, pk, pk_stride, sig, sig_stride, msg, moff,
+                                            mlen, status, ws, stream), "cuspx_verify_device")
+
+
+def keygen_device(pid, num, seed, seed_stride, pk, sk, ws, stream=None):
+    return _check(lib().cuspx_keygen_device(pid, num, seed, seed_stride, pk, sk, ws, stream),
+                  "cuspx_keygen_device")
+
+
+def profile_enable(on=True):
+    lib().cuspx_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().cuspx_profile_reset()
+
+
+def profile_read():
+    """{kernel: (total_ms, launches)} since the last reset."""
+    cap = 64
+    names = ctypes.create_string_buffer(4096)
+    ms = (ctypes.c_float * cap)()
+    cnt = (ctypes.c_int * cap)()
+    k = lib().cuspx_profile_read(names, 4096, ms, cnt, cap)
+    keys = names.value.decode().split("\n")[:k]
+    return {keys[i]: (ms[i], cnt[i]) for i in range(k)}

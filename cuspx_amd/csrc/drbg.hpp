@@ -1,0 +1,9 @@
+// drbg.hpp -- host AES-256-CTR DRBG (rng.cu:116-241 semantics).
+#pragma once
+#include <stdint.h>
+
+namespace spx {
+void aes256_ecb(const uint8_t key[32], const uint8_t in[16], uint8_t out[16]);
+void drbg_init(const uint8_t entropy[48], const uint8_t* pers);
+void drbg_bytes(uint8_t* x, uint64_t xlen);
+}  // namespace spx

@@ -1,0 +1,692 @@
+// engine.cpp -- C ABI of libcuspx.so: the api.h surface of the reference
+// (crypto_sign_* / face_mdp_* / randombytes), runtime-parameterised batch
+// entry points with per-item status, device-resident entry points, and the
+// kernel-timing hooks.  All device work goes through the HIP kernels of the
+// set_*.hip translation units; there is no host fallback for any hash.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cuspx.h"
+#include "drbg.hpp"
+#include "engine.hpp"
+#include "profile.hpp"
+
+extern "C" {
+const spx::SetOps* cuspx_set_sha2_128s(void);
+const spx::SetOps* cuspx_set_sha2_128f(void);
+const spx::SetOps* cuspx_set_sha2_192s(void);
+const spx::SetOps* cuspx_set_sha2_192f(void);
+const spx::SetOps* cuspx_set_sha2_256s(void);
+const spx::SetOps* cuspx_set_sha2_256f(void);
+const spx::SetOps* cuspx_set_shake256_128s(void);
+const spx::SetOps* cuspx_set_shake256_128f(void);
+const spx::SetOps* cuspx_set_shake256_192s(void);
+const spx::SetOps* cuspx_set_shake256_192f(void);
+const spx::SetOps* cuspx_set_shake256_256s(void);
+const spx::SetOps* cuspx_set_shake256_256f(void);
+}
+
+namespace spx {
+
+static const SetOps* (*const kSetFns[12])(void) = {
+    cuspx_set_sha2_128s,     cuspx_set_sha2_128f,     cuspx_set_sha2_192s,
+    cuspx_set_sha2_192f,     cuspx_set_sha2_256s,     cuspx_set_sha2_256f,
+    cuspx_set_shake256_128s, cuspx_set_shake256_128f, cuspx_set_shake256_192s,
+    cuspx_set_shake256_192f, cuspx_set_shake256_256s, cuspx_set_shake256_256f};
+
+int num_sets() { return 12; }
+
+const SetOps* get_set(int pid) {
+    if (pid < 0 || pid >= 12) return nullptr;
+    return kSetFns[pid]();
+}
+
+// ------------------------------------------------------------ error text
+
+static thread_local char g_err[256] = "";
+
+static int fail(int code, const char* fmt, const char* what = "") {
+    snprintf(g_err, sizeof(g_err), fmt, what);
+    return code;
+}
+
+#define HIP_OK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) return fail(kErrHip, "HIP error: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------ the device
+
+struct Device {
+    std::mutex mu;
+    bool ready = false;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    void* dbuf = nullptr;   // device arena for host-buffer calls (grow-only)
+    size_t dcap = 0;
+    uint8_t* hbuf = nullptr;  // pinned staging (grow-only)
+    size_t hcap = 0;
+    size_t budget = 0;      // bytes one host-buffer call may use on the device
+
+    int init() {
+        if (ready) return kOk;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        size_t fr = 0, tot = 0;
+        HIP_OK(hipMemGetInfo(&fr, &tot));
+        const char* env = getenv("CUSPX_HBM_BUDGET_MB");
+        budget = env ? (size_t)atoll(env) << 20 : fr / 2;
+        ready = true;
+        return kOk;
+    }
+    int ensure(size_t dbytes, size_t hbytes) {
+        if (dbytes > dcap) {
+            if (dbuf) HIP_OK(hipFree(dbuf));
+            dbuf = nullptr;
+            dcap = 0;
+            HIP_OK(hipMalloc(&dbuf, dbytes));
+            dcap = dbytes;
+        }
+        if (hbytes > hcap) {
+            if (hbuf) HIP_OK(hipHostFree(hbuf));
+            hbuf = nullptr;
+            hcap = 0;
+            HIP_OK(hipHostMalloc((void**)&hbuf, hbytes, hipHostMallocDefault));
+            hcap = hbytes;
+        }
+        return kOk;
+    }
+};
+
+static Device& device() {
+    static Device d;
+    return d;
+}
+
+// ------------------------------------------------------------ settings
+
+static int g_default_pid = -1;
+static int g_debug_mode = 1;  // the reference ships with DEBUG_MODE (all_option.h:74)
+
+static int default_pid() {
+    if (g_default_pid < 0) {
+        g_default_pid = 1;  // sha2-128f
+        const char* env = getenv("CUSPX_PARAM");
+        if (env) {
+            for (int i = 0; i < 12; i++)
+                if (strcmp(env, get_set(i)->name) == 0) g_default_pid = i;
+        }
+    }
+    return g_default_pid;
+}
+
+static inline uint64_t al16(uint64_t x) { return (x + 15) & ~uint64_t(15); }
+
+// ------------------------------------------------------- profiling hooks
+
+struct Prof {
+    std::mutex mu;
+    bool on = false;
+    struct Mark {
+        const char* name;
+        hipEvent_t ev;
+    };
+    std::vector<std::vector<Mark>> calls;
+    std::map<hipStream_t, size_t> open;
+    std::map<std::string, std::pair<double, int>> acc;
+};
+static Prof& prof() {
+    static Prof p;
+    return p;
+}
+
+void prof_begin(hipStream_t s) {
+    Prof& p = prof();
+    if (!p.on) return;
+    std::lock_guard<std::mutex> lk(p.mu);
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, s);
+    p.calls.push_back({{"", e}});
+    p.open[s] = p.calls.size() - 1;
+}
+
+void prof_mark(hipStream_t s, const char* kernel) {
+    Prof& p = prof();
+    if (!p.on) return;
+    std::lock_guard<std::mutex> lk(p.mu);
+    auto it = p.open.find(s);
+    if (it == p.open.end()) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, s);
+    p.calls[it->second].push_back({kernel, e});
+}
+
+// ------------------------------------------------ host-buffer batch calls
+
+struct MsgPack {
+    std::vector<uint64_t> off;  // device offsets, 16-byte aligned
+    uint64_t bytes = 0;
+};
+
+static MsgPack pack_msgs(uint32_t num, const uint32_t* mlen) {
+    MsgPack mp;
+    mp.off.resize(num);
+    uint64_t o = 0;
+    for (uint32_t i = 0; i < num; i++) {
+        mp.off[i] = o;
+        o += al16(mlen[i]);
+    }
+    mp.bytes = o + 16;
+    return mp;
+}
+
+// sign `num` items: sk_i = sk + i*sk_stride, message i at msg + moff[i]
+static int sign_host(const SetOps* o, uint32_t num, const uint8_t* sk, uint64_t sk_stride,
+                     const uint8_t* msg, const uint64_t* moff, const uint32_t* mlen,
+                     const uint8_t* optrand, uint8_t* sig, uint64_t sig_stride) {
+    if (num == 0) return kOk;
+    if (!sk || !sig || !mlen || !moff || !optrand) return fail(kErrArg, "null argument%s");
+    Device& D = device();
+    std::lock_guard<std::mutex> lk(D.mu);
+    int rc = D.init();
+    if (rc) return rc;
+    MsgPack mp = pack_msgs(num, mlen);
+    const uint64_t n = o->n;
+    const uint64_t in_bytes = al16((uint64_t)num * o->sk_bytes) + mp.bytes + al16(8ull * num) +
+                              al16(4ull * num) + al16((uint64_t)num * n);
+    const uint64_t out_bytes = al16((uint64_t)num * o->sig_bytes);
+    const uint64_t ws = o->workspace(num, kOpSign);
+    if (in_bytes + out_bytes + ws + 512 > D.budget)
+        return fail(kErrBudget, "sign batch exceeds the device-memory budget%s");
+    rc = D.ensure(in_bytes + out_bytes + ws + 512, in_bytes > out_bytes ? in_bytes : out_bytes);
+    if (rc) return rc;
+    uint8_t* h = D.hbuf;
+    uint8_t* p = h;
+    for (uint32_t i = 0; i < num; i++) memcpy(p + (uint64_t)i * o->sk_bytes, sk + i * sk_stride, o->sk_bytes);
+    p += al16((uint64_t)num * o->sk_bytes);
+    uint8_t* pm = p;
+    memset(pm, 0, mp.bytes);
+    for (uint32_t i = 0; i < num; i++) memcpy(pm + mp.off[i], msg + moff[i], mlen[i]);
+    p += mp.bytes;
+    memcpy(p, mp.off.data(), 8ull * num);
+    p += al16(8ull * num);
+    memcpy(p, mlen, 4ull * num);
+    p += al16(4ull * num);
+    memcpy(p, optrand, (uint64_t)num * n);
+    uint8_t* d = (uint8_t*)D.dbuf;
+    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, D.stream));
+    DevBatch b{};
+    b.n = num;
+    b.sk = d;
+    b.sk_stride = o->sk_bytes;
+    uint8_t* q = d + al16((uint64_t)num * o->sk_bytes);
+    b.msg = q;
+    q += mp.bytes;
+    b.moff = (const uint64_t*)q;
+    q += al16(8ull * num);
+    b.mlen = (const uint32_t*)q;
+    q += al16(4ull * num);
+    b.optrand = q;
+    uint8_t* dsig = d + in_bytes;
+    b.sig = dsig;
+    b.sig_stride = o->sig_bytes;
+    b.ws = dsig + out_bytes;
+    rc = o->sign(b, D.stream);
+    if (rc) return fail(rc, "kernel launch failed%s");
+    HIP_OK(hipMemcpyAsync(h, dsig, (uint64_t)num * o->sig_bytes, hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+    for (uint32_t i = 0; i < num; i++)
+        memcpy(sig + i * sig_stride, h + (uint64_t)i * o->sig_bytes, o->sig_bytes);
+    return kOk;
+}
+
+static int verify_host(const SetOps* o, uint32_t num, const uint8_t* pk, uint64_t pk_stride,
+                       const uint8_t* sig, uint64_t sig_stride, const uint8_t* msg,
+                       const uint64_t* moff, const uint32_t* mlen, int32_t* status) {
+    if (num == 0) return kOk;
+    if (!pk || !sig || !mlen || !moff || !status) return fail(kErrArg, "null argument%s");
+    Device& D = device();
+    std::lock_guard<std::mutex> lk(D.mu);
+    int rc = D.init();
+    if (rc) return rc;
+    MsgPack mp = pack_msgs(num, mlen);
+    const uint64_t in_bytes = al16((uint64_t)num * o->pk_bytes) +
+                              al16((uint64_t)num * o->sig_bytes) + mp.bytes + al16(8ull * num) +
+                              al16(4ull * num);
+    const uint64_t out_bytes = al16(4ull * num);
+    const uint64_t ws = o->workspace(num, kOpVerify);
+    if (in_bytes + out_bytes + ws + 512 > D.budget)
+        return fail(kErrBudget, "verify batch exceeds the device-memory budget%s");
+    rc = D.ensure(in_bytes + out_bytes + ws + 512, in_bytes);
+    if (rc) return rc;
+    uint8_t* h = D.hbuf;
+    uint8_t* p = h;
+    for (uint32_t i = 0; i < num; i++) memcpy(p + (uint64_t)i * o->pk_bytes, pk + i * pk_stride, o->pk_bytes);
+    p += al16((uint64_t)num * o->pk_bytes);
+    for (uint32_t i = 0; i < num; i++)
+        memcpy(p + (uint64_t)i * o->sig_bytes, sig + i * sig_stride, o->sig_bytes);
+    p += al16((uint64_t)num * o->sig_bytes);
+    memset(p, 0, mp.bytes);
+    for (uint32_t i = 0; i < num; i++) memcpy(p + mp.off[i], msg + moff[i], mlen[i]);
+    p += mp.bytes;
+    memcpy(p, mp.off.data(), 8ull * num);
+    p += al16(8ull * num);
+    memcpy(p, mlen, 4ull * num);
+    uint8_t* d = (uint8_t*)D.dbuf;
+    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, D.stream));
+    DevBatch b{};
+    b.n = num;
+    b.pk = d;
+    b.pk_stride = o->pk_bytes;
+    uint8_t* q = d + al16((uint64_t)num * o->pk_bytes);
+    b.sig = q;
+    b.sig_stride = o->sig_bytes;
+    q += al16((uint64_t)num * o->sig_bytes);
+    b.msg = q;
+    q += mp.bytes;
+    b.moff = (const uint64_t*)q;
+    q += al16(8ull * num);
+    b.mlen = (const uint32_t*)q;
+    int32_t* dst = (int32_t*)(d + in_bytes);
+    b.status = dst;
+    b.ws = (uint8_t*)dst + out_bytes;
+    rc = o->verify(b, D.stream);
+    if (rc) return fail(rc, "kernel launch failed%s");
+    HIP_OK(hipMemcpyAsync(h, dst, 4ull * num, hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+    memcpy(status, h, 4ull * num);
+    for (uint32_t i = 0; i < num; i++)
+        if (status[i] != 0) return kErrVerify;
+    return kOk;
+}
+
+static int keygen_host(const SetOps* o, uint32_t num, const uint8_t* seeds, uint64_t seed_stride,
+                       uint8_t* pk, uint8_t* sk) {
+    if (num == 0) return kOk;
+    if (!seeds || !pk || !sk) return fail(kErrArg, "null argument%s");
+    Device& D = device();
+    std::lock_guard<std::mutex> lk(D.mu);
+    int rc = D.init();
+    if (rc) return rc;
+    const uint64_t sb = 3ull * o->n;
+    const uint64_t in_bytes = al16(num * sb);
+    const uint64_t out_bytes = al16((uint64_t)num * o->pk_bytes) + al16((uint64_t)num * o->sk_bytes);
+    const uint64_t ws = o->workspace(num, kOpKeygen);
+    if (in_bytes + out_bytes + ws + 512 > D.budget)
+        return fail(kErrBudget, "keygen batch exceeds the device-memory budget%s");
+    rc = D.ensure(in_bytes + out_bytes + ws + 512, in_bytes > out_bytes ? in_bytes : out_bytes);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < num; i++) memcpy(D.hbuf + i * sb, seeds + i * seed_stride, sb);
+    uint8_t* d = (uint8_t*)D.dbuf;
+    HIP_OK(hipMemcpyAsync(d, D.hbuf, num * sb, hipMemcpyHostToDevice, D.stream));
+    DevBatch b{};
+    b.n = num;
+    b.seed = d;
+    b.seed_stride = (uint32_t)sb;
+    b.pk_out = d + in_bytes;
+    b.sk_out = b.pk_out + al16((uint64_t)num * o->pk_bytes);
+    b.ws = d + in_bytes + out_bytes;
+    rc = o->keygen(b, D.stream);
+    if (rc) return fail(rc, "kernel launch failed%s");
+    HIP_OK(hipMemcpyAsync(D.hbuf, b.pk_out, out_bytes, hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+    memcpy(pk, D.hbuf, (uint64_t)num * o->pk_bytes);
+    memcpy(sk, D.hbuf + al16((uint64_t)num * o->pk_bytes), (uint64_t)num * o->sk_bytes);
+    return kOk;
+}
+
+// draw the per-item optrand the way crypto_sign_signature does
+// (sign.cu:1950-1954): n DRBG bytes, replaced by 0xad in debug mode
+static void draw_optrand(uint8_t* out, uint32_t num, int n) {
+    for (uint32_t i = 0; i < num; i++) {
+        drbg_bytes(out + (uint64_t)i * n, n);
+        if (g_debug_mode) memset(out + (uint64_t)i * n, 0xad, n);
+    }
+}
+
+// draw keygen seeds the way crypto_sign_keypair does (sign.cu:342-352)
+static void draw_seed(uint8_t* out, int n) {
+    drbg_bytes(out, 3ull * n);
+    if (g_debug_mode)
+        for (int i = 0; i < 3 * n; i++) out[i] = (uint8_t)i;
+}
+
+}  // namespace spx
+
+using namespace spx;
+
+// =====================================================================
+// C ABI
+// =====================================================================
+
+extern "C" {
+
+int cuspx_num_params(void) { return 12; }
+
+int cuspx_param_id(const char* name) {
+    if (!name) return kErrArg;
+    for (int i = 0; i < 12; i++)
+        if (strcmp(name, get_set(i)->name) == 0) return i;
+    return kErrParam;
+}
+
+int cuspx_get_param_info(int pid, cuspx_param_info* out) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    if (!out) return kErrArg;
+    memset(out, 0, sizeof(*out));
+    out->pid = pid;
+    out->hash = o->hash;
+    out->n = o->n;
+    out->full_h = o->full_h;
+    out->d = o->d;
+    out->a = o->a;
+    out->k = o->k;
+    out->len = o->len;
+    out->sig_bytes = o->sig_bytes;
+    out->pk_bytes = o->pk_bytes;
+    out->sk_bytes = o->sk_bytes;
+    out->seed_bytes = 3 * o->n;
+    snprintf(out->name, sizeof(out->name), "%s", o->name);
+    return kOk;
+}
+
+int cuspx_set_default_param(int pid) {
+    if (!get_set(pid)) return kErrParam;
+    g_default_pid = pid;
+    return kOk;
+}
+int cuspx_get_default_param(void) { return default_pid(); }
+void cuspx_set_debug_mode(int on) { g_debug_mode = on ? 1 : 0; }
+int cuspx_get_debug_mode(void) { return g_debug_mode; }
+const char* cuspx_last_error(void) { return g_err; }
+
+void randombytes_init(unsigned char* entropy_input, unsigned char* personalization_string,
+                      int security_strength) {
+    (void)security_strength;
+    drbg_init(entropy_input, personalization_string);
+}
+
+int randombytes(unsigned char* x, unsigned long long xlen) {
+    drbg_bytes(x, xlen);
+    return 0;
+}
+
+unsigned long long crypto_sign_secretkeybytes(void) { return get_set(default_pid())->sk_bytes; }
+unsigned long long crypto_sign_publickeybytes(void) { return get_set(default_pid())->pk_bytes; }
+unsigned long long crypto_sign_bytes(void) { return get_set(default_pid())->sig_bytes; }
+unsigned long long crypto_sign_seedbytes(void) { return 3ull * get_set(default_pid())->n; }
+
+int crypto_sign_seed_keypair(unsigned char* pk, unsigned char* sk, const unsigned char* seed) {
+    return keygen_host(get_set(default_pid()), 1, seed, 0, pk, sk);
+}
+
+int crypto_sign_keypair(unsigned char* pk, unsigned char* sk) {
+    const SetOps* o = get_set(default_pid());
+    uint8_t seed[96];
+    draw_seed(seed, o->n);
+    return keygen_host(o, 1, seed, 0, pk, sk);
+}
+
+int crypto_sign_signature(uint8_t* sig, size_t* siglen, const uint8_t* m, size_t mlen,
+                          const uint8_t* sk) {
+    const SetOps* o = get_set(default_pid());
+    uint8_t opt[32];
+    draw_optrand(opt, 1, o->n);
+    const uint64_t off = 0;
+    const uint32_t ml = (uint32_t)mlen;
+    int rc = sign_host(o, 1, sk, 0, m, &off, &ml, opt, sig, 0);
+    if (rc == kOk && siglen) *siglen = o->sig_bytes;
+    return rc;
+}
+
+int crypto_sign_verify(const uint8_t* sig, size_t siglen, const uint8_t* m, size_t mlen,
+                       const uint8_t* pk) {
+    const SetOps* o = get_set(default_pid());
+    if (siglen != (size_t)o->sig_bytes) return kErrVerify;  // sign.cu:3065-3067
+    const uint64_t off = 0;
+    const uint32_t ml = (uint32_t)mlen;
+    int32_t st = -1;
+    int rc = verify_host(o, 1, pk, 0, sig, 0, m, &off, &ml, &st);
+    return rc == kOk ? kOk : (rc == kErrVerify ? kErrVerify : rc);
+}
+
+int crypto_sign(unsigned char* sm, unsigned long long* smlen, const unsigned char* m,
+                unsigned long long mlen, const unsigned char* sk) {
+    const SetOps* o = get_set(default_pid());
+    std::vector<uint8_t> mc(m, m + mlen);  // m may alias sm + SPX_BYTES
+    size_t siglen = 0;
+    int rc = crypto_sign_signature(sm, &siglen, mc.data(), mlen, sk);
+    if (rc) return rc;
+    memmove(sm + o->sig_bytes, mc.data(), mlen);
+    *smlen = siglen + mlen;
+    return kOk;
+}
+
+int crypto_sign_open(unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
+                     unsigned long long smlen, const unsigned char* pk) {
+    const SetOps* o = get_set(default_pid());
+    if (smlen < (unsigned long long)o->sig_bytes) {  // sign.cu:6061-6065
+        memset(m, 0, smlen);
+        *mlen = 0;
+        return kErrVerify;
+    }
+    const unsigned long long ml = smlen - o->sig_bytes;
+    int rc = crypto_sign_verify(sm, o->sig_bytes, sm + o->sig_bytes, ml, pk);
+    if (rc) {
+        memset(m, 0, smlen);
+        *mlen = 0;
+        return rc == kErrVerify ? kErrVerify : rc;
+    }
+    memmove(m, sm + o->sig_bytes, ml);
+    *mlen = ml;
+    return kOk;
+}
+
+int face_mdp_crypto_sign_keypair(unsigned char* pk, unsigned char* sk, unsigned int num) {
+    const SetOps* o = get_set(default_pid());
+    std::vector<uint8_t> seeds(3ull * o->n * num);
+    for (unsigned i = 0; i < num; i++) draw_seed(seeds.data() + 3ull * o->n * i, o->n);
+    return keygen_host(o, num, seeds.data(), 3ull * o->n, pk, sk);
+}
+
+int face_mdp_crypto_sign(unsigned char* sm, unsigned long long* smlen, const unsigned char* m,
+                         unsigned long long mlen, const unsigned char* sk, unsigned int num) {
+    const SetOps* o = get_set(default_pid());
+    const uint64_t stride = (uint64_t)o->sig_bytes + 32;  // SPX_SM_BYTES (params.h:150)
+    std::vector<uint64_t> off(num);
+    std::vector<uint32_t> ml(num, (uint32_t)mlen);
+    for (unsigned i = 0; i < num; i++) off[i] = 32ull * i;  // SPX_MLEN stride (sign.cu:3500)
+    std::vector<uint8_t> opt((uint64_t)num * o->n);
+    draw_optrand(opt.data(), num, o->n);
+    int rc = sign_host(o, num, sk, o->sk_bytes, m, off.data(), ml.data(), opt.data(), sm, stride);
+    if (rc) return rc;
+    for (unsigned i = 0; i < num; i++) memcpy(sm + i * stride + o->sig_bytes, m + 32ull * i, mlen);
+    *smlen = stride;
+    return kOk;
+}
+
+int face_mdp_crypto_sign_open(unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
+                              unsigned long long smlen, const unsigned char* pk, unsigned int num) {
+    const SetOps* o = get_set(default_pid());
+    const uint64_t stride = (uint64_t)o->sig_bytes + 32;
+    if (smlen < (unsigned long long)o->sig_bytes) {
+        memset(m, 0, 32ull * num);
+        *mlen = 0;
+        return kErrVerify;
+    }
+    const uint32_t ml = (uint32_t)(smlen - o->sig_bytes);
+    std::vector<uint64_t> off(num);
+    std::vector<uint32_t> mls(num, ml);
+    for (unsigned i = 0; i < num; i++) off[i] = i * stride + o->sig_bytes;
+    std::vector<int32_t> st(num, -1);
+    int rc = verify_host(o, num, pk, o->pk_bytes, sm, stride, sm, off.data(), mls.data(), st.data());
+    if (rc != kOk && rc != kErrVerify) return rc;
+    for (unsigned i = 0; i < num; i++) {
+        if (st[i] == 0) memcpy(m + 32ull * i, sm + i * stride + o->sig_bytes, ml < 32 ? ml : 32);
+        else memset(m + 32ull * i, 0, 32);
+    }
+    *mlen = ml;
+    return rc;
+}
+
+int spx_batch_keygen(int pid, uint32_t num, const uint8_t* seeds, uint8_t* pk, uint8_t* sk) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    return keygen_host(o, num, seeds, 3ull * o->n, pk, sk);
+}
+
+int spx_batch_sign(int pid, uint32_t num, const uint8_t* sk, const uint8_t* msg,
+                   const uint64_t* moff, const uint32_t* mlen, const uint8_t* optrand,
+                   uint8_t* sig) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    std::vector<uint8_t> opt;
+    if (!optrand) {
+        opt.resize((uint64_t)num * o->n);
+        draw_optrand(opt.data(), num, o->n);
+        optrand = opt.data();
+    }
+    return sign_host(o, num, sk, o->sk_bytes, msg, moff, mlen, optrand, sig, o->sig_bytes);
+}
+
+int spx_batch_verify(int pid, uint32_t num, const uint8_t* pk, const uint8_t* sig,
+                     const uint8_t* msg, const uint64_t* moff, const uint32_t* mlen,
+                     int32_t* status) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    return verify_host(o, num, pk, o->pk_bytes, sig, o->sig_bytes, msg, moff, mlen, status);
+}
+
+size_t cuspx_workspace_bytes(int pid, uint32_t num, int op) {
+    const SetOps* o = get_set(pid);
+    if (!o || op < 0 || op > 2) return 0;
+    return o->workspace(num, op);
+}
+
+static hipStream_t pick_stream(void* stream) {
+    if (stream) return (hipStream_t)stream;
+    Device& D = device();
+    std::lock_guard<std::mutex> lk(D.mu);
+    D.init();
+    return D.stream;
+}
+
+int cuspx_sign_device(int pid, uint32_t num, const uint8_t* sk, uint32_t sk_stride,
+                      const uint8_t* msg, const uint64_t* moff, const uint32_t* mlen,
+                      const uint8_t* optrand, uint8_t* sig, uint64_t sig_stride, void* ws,
+                      void* stream) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    if (num == 0) return kOk;
+    if (!sk || !msg || !moff || !mlen || !optrand || !sig || !ws) return fail(kErrArg, "null argument%s");
+    DevBatch b{};
+    b.n = num;
+    b.sk = sk;
+    b.sk_stride = sk_stride;
+    b.msg = msg;
+    b.moff = moff;
+    b.mlen = mlen;
+    b.optrand = optrand;
+    b.sig = sig;
+    b.sig_stride = sig_stride;
+    b.ws = ws;
+    return o->sign(b, pick_stream(stream));
+}
+
+int cuspx_verify_device(int pid, uint32_t num, const uint8_t* pk, uint32_t pk_stride,
+                        const uint8_t* sig, uint64_t sig_stride, const uint8_t* msg,
+                        const uint64_t* moff, const uint32_t* mlen, int32_t* status, void* ws,
+                        void* stream) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    if (num == 0) return kOk;
+    if (!pk || !sig || !msg || !moff || !mlen || !status || !ws) return fail(kErrArg, "null argument%s");
+    DevBatch b{};
+    b.n = num;
+    b.pk = pk;
+    b.pk_stride = pk_stride;
+    b.sig = const_cast<uint8_t*>(sig);
+    b.sig_stride = sig_stride;
+    b.msg = msg;
+    b.moff = moff;
+    b.mlen = mlen;
+    b.status = status;
+    b.ws = ws;
+    return o->verify(b, pick_stream(stream));
+}
+
+int cuspx_keygen_device(int pid, uint32_t num, const uint8_t* seed, uint32_t seed_stride,
+                        uint8_t* pk, uint8_t* sk, void* ws, void* stream) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    if (num == 0) return kOk;
+    if (!seed || !pk || !sk || !ws) return fail(kErrArg, "null argument%s");
+    DevBatch b{};
+    b.n = num;
+    b.seed = seed;
+    b.seed_stride = seed_stride;
+    b.pk_out = pk;
+    b.sk_out = sk;
+    b.ws = ws;
+    return o->keygen(b, pick_stream(stream));
+}
+
+void cuspx_profile_enable(int on) {
+    Prof& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.on = on != 0;
+}
+
+void cuspx_profile_reset(void) {
+    Prof& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (auto& c : p.calls)
+        for (auto& m : c) hipEventDestroy(m.ev);
+    p.calls.clear();
+    p.open.clear();
+    p.acc.clear();
+}
+
+int cuspx_profile_read(char* names, int names_cap, float* ms, int* count, int cap) {
+    Prof& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (auto& c : p.calls) {
+        for (size_t i = 0; i < c.size(); i++) hipEventSynchronize(c[i].ev);
+        for (size_t i = 1; i < c.size(); i++) {
+            float t = 0.f;
+            hipEventElapsedTime(&t, c[i - 1].ev, c[i].ev);
+            auto& a = p.acc[c[i].name];
+            a.first += t;
+            a.second += 1;
+        }
+        for (auto& m : c) hipEventDestroy(m.ev);
+    }
+    p.calls.clear();
+    p.open.clear();
+    int k = 0;
+    std::string all;
+    for (auto& kv : p.acc) {
+        if (k >= cap) break;
+        ms[k] = (float)kv.second.first;
+        count[k] = kv.second.second;
+        all += kv.first;
+        all += '\n';
+        k++;
+    }
+    if (names && names_cap > 0) snprintf(names, names_cap, "%s", all.c_str());
+    return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// engine.hpp -- host-side types shared by the per-set launchers and the
+// C-ABI layer.  No torch types: plain pointers, sizes and HIP streams.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace spx {
+
+enum { kOpSign = 0, kOpVerify = 1, kOpKeygen = 2 };
+
+// status codes of every C-ABI entry point
+enum {
+    kOk = 0,
+    kErrVerify = -1,   // verification failed (api.h semantics)
+    kErrParam = -2,    // unknown parameter-set id
+    kErrHip = -3,      // HIP runtime / launch error
+    kErrArg = -4,      // bad argument (null pointer, size)
+    kErrBudget = -5,   // request exceeds the device-memory budget
+};
+
+// One batch on the device.  Every pointer is device memory.
+struct DevBatch {
+    uint32_t n;
+    const uint8_t* sk;
+    uint32_t sk_stride;
+    const uint8_t* pk;
+    uint32_t pk_stride;
+    const uint8_t* seed;
+    uint32_t seed_stride;
+    const uint8_t* msg;      // msg + moff[i]: 4-byte aligned, readable to a 4-byte boundary
+    const uint64_t* moff;
+    const uint32_t* mlen;
+    const uint8_t* optrand;  // [n][param n]
+    uint8_t* sig;
+    uint64_t sig_stride;
+    int32_t* status;
+    uint8_t* pk_out;
+    uint8_t* sk_out;
+    void* ws;                // workspace of workspace(n, op) bytes
+};
+
+struct SetOps {
+    const char* name;
+    int pid;
+    int hash, n, full_h, d, a, k, len;
+    int sig_bytes, pk_bytes, sk_bytes;
+    size_t (*workspace)(uint32_t n, int op);
+    int (*sign)(const DevBatch&, hipStream_t);
+    int (*verify)(const DevBatch&, hipStream_t);
+    int (*keygen)(const DevBatch&, hipStream_t);
+};
+
+const SetOps* get_set(int pid);
+int num_sets();
+
+}  // namespace spx

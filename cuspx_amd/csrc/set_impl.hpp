@@ -1,0 +1,168 @@
+// set_impl.hpp -- host-side launch sequences for one parameter set.
+// Each set_<name>.hip instantiates SetImpl<P, J> and registers its SetOps.
+#pragma once
+#include "engine.hpp"
+#include "profile.hpp"
+#include "spx_kernels.hpp"
+
+namespace spx {
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// workspace carve-up shared by sign / verify / keygen
+template <class P>
+struct Layout {
+    static constexpr uint64_t al(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+    static uint32_t ht_block(void) { return (1u << P::th) > 256 ? (1u << P::th) : 256u; }
+    static uint32_t ht_trees_per_block(void) { return ht_block() >> P::th; }
+    static uint32_t ht_blocks(uint32_t trees) { return cdiv(trees, ht_trees_per_block()); }
+    static uint64_t tl_lanes(uint32_t trees) { return (uint64_t)ht_blocks(trees) * ht_block(); }
+    static uint64_t bytes(uint32_t n, int op) {
+        uint64_t b = al((uint64_t)n * sizeof(Plan));
+        b += al((uint64_t)n * P::k * P::n);   // fors roots
+        b += al((uint64_t)n * P::n);          // fors pk
+        if (op == kOpSign) {
+            b += al((uint64_t)n * P::d * P::n);                    // layer roots
+            b += al((uint64_t)n * P::d * P::len * 16 * P::n);      // chain capture
+        }
+        if (P::hash == kShake && op != kOpVerify) {
+            const uint32_t trees = op == kOpSign ? n * P::d : n;
+            b += al(tl_lanes(trees) * P::len * P::nw * 4);
+        }
+        return b + 256;
+    }
+    static void carve(BatchArgs& A, uint8_t* ws, uint32_t n, int op) {
+        uint8_t* p = (uint8_t*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+        A.plan = (Plan*)p;
+        p += al((uint64_t)n * sizeof(Plan));
+        A.fors_roots = p;
+        p += al((uint64_t)n * P::k * P::n);
+        A.fors_pk = p;
+        p += al((uint64_t)n * P::n);
+        A.roots = nullptr;
+        A.cap = nullptr;
+        A.tl_buf = nullptr;
+        A.tl_lanes = 0;
+        if (op == kOpSign) {
+            A.roots = p;
+            p += al((uint64_t)n * P::d * P::n);
+            A.cap = p;
+            p += al((uint64_t)n * P::d * P::len * 16 * P::n);
+        }
+        if (P::hash == kShake && op != kOpVerify) {
+            const uint32_t trees = op == kOpSign ? n * P::d : n;
+            A.tl_buf = (uint32_t*)p;
+            A.tl_lanes = (uint32_t)tl_lanes(trees);
+        }
+    }
+};
+
+template <class P, int J>
+struct SetImpl {
+    using Lay = Layout<P>;
+
+    static size_t workspace(uint32_t n, int op) { return (size_t)Lay::bytes(n, op); }
+
+    static int sign(const DevBatch& b, hipStream_t s) {
+        BatchArgs A{};
+        A.sk = b.sk;
+        A.sk_stride = b.sk_stride;
+        A.msg = b.msg;
+        A.moff = b.moff;
+        A.mlen = b.mlen;
+        A.optrand = b.optrand;
+        A.sig = b.sig;
+        A.sig_stride = b.sig_stride;
+        A.n_req = b.n;
+        Lay::carve(A, (uint8_t*)b.ws, b.n, kOpSign);
+        const uint32_t n = b.n;
+        constexpr uint32_t fors_tpw = 256u >> (P::a - J);
+        prof_begin(s);
+        hipLaunchKernelGGL((k_prep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+        prof_mark(s, "k_prep");
+        hipLaunchKernelGGL((k_fors<P, J>), dim3(cdiv((uint64_t)n * P::k, fors_tpw)), dim3(256), 0,
+                           s, A);
+        prof_mark(s, "k_fors");
+        hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+        prof_mark(s, "k_fors_pk");
+        hipLaunchKernelGGL((k_ht<P, 0>), dim3(Lay::ht_blocks(n * P::d)), dim3(Lay::ht_block()), 0,
+                           s, A);
+        prof_mark(s, "k_ht");
+        hipLaunchKernelGGL((k_wots_gather<P>), dim3(cdiv((uint64_t)n * P::d * P::len, 256)),
+                           dim3(256), 0, s, A);
+        prof_mark(s, "k_wots_gather");
+        return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+    }
+
+    static int verify(const DevBatch& b, hipStream_t s) {
+        BatchArgs A{};
+        A.pk = b.pk;
+        A.pk_stride = b.pk_stride;
+        A.msg = b.msg;
+        A.moff = b.moff;
+        A.mlen = b.mlen;
+        A.sig = b.sig;
+        A.sig_stride = b.sig_stride;
+        A.status = b.status;
+        A.n_req = b.n;
+        Lay::carve(A, (uint8_t*)b.ws, b.n, kOpVerify);
+        const uint32_t n = b.n;
+        constexpr uint32_t groups = 256 / P::len;
+        prof_begin(s);
+        hipLaunchKernelGGL((k_vprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+        prof_mark(s, "k_vprep");
+        hipLaunchKernelGGL((k_vfors<P>), dim3(cdiv((uint64_t)n * P::k, 64)), dim3(64), 0, s, A);
+        prof_mark(s, "k_vfors");
+        hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+        prof_mark(s, "k_fors_pk");
+        hipLaunchKernelGGL((k_vht<P>), dim3(cdiv(n, groups)), dim3(256), 0, s, A);
+        prof_mark(s, "k_vht");
+        return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+    }
+
+    static int keygen(const DevBatch& b, hipStream_t s) {
+        BatchArgs A{};
+        A.seed = b.seed;
+        A.seed_stride = b.seed_stride;
+        A.pk_out = b.pk_out;
+        A.sk_out = b.sk_out;
+        A.n_req = b.n;
+        Lay::carve(A, (uint8_t*)b.ws, b.n, kOpKeygen);
+        const uint32_t n = b.n;
+        prof_begin(s);
+        hipLaunchKernelGGL((k_kprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+        prof_mark(s, "k_kprep");
+        hipLaunchKernelGGL((k_ht<P, 1>), dim3(Lay::ht_blocks(n)), dim3(Lay::ht_block()), 0, s, A);
+        prof_mark(s, "k_ht_keygen");
+        return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+    }
+
+    static SetOps ops(const char* name, int pid) {
+        SetOps o{};
+        o.name = name;
+        o.pid = pid;
+        o.hash = P::hash;
+        o.n = P::n;
+        o.full_h = P::full_h;
+        o.d = P::d;
+        o.a = P::a;
+        o.k = P::k;
+        o.len = P::len;
+        o.sig_bytes = P::sig_bytes;
+        o.pk_bytes = P::pk_bytes;
+        o.sk_bytes = P::sk_bytes;
+        o.workspace = &workspace;
+        o.sign = &sign;
+        o.verify = &verify;
+        o.keygen = &keygen;
+        return o;
+    }
+};
+
+}  // namespace spx
+
+#define SPX_REGISTER_SET(SYM, NAME, PID, PARAMS, J)                        \
+    extern "C" const spx::SetOps* SYM(void) {                              \
+        static const spx::SetOps o = spx::SetImpl<PARAMS, J>::ops(NAME, PID); \
+        return &o;                                                         \
+    }

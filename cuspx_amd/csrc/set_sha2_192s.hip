@@ -1,0 +1,5 @@
+// sha2-192s: n=24 h=63 d=7 a=14 k=17 (params.h:33-79); FORS: 2^6 leaves per lane
+#include "set_impl.hpp"
+
+using SetParams = spx::Params<spx::kSha2, 24, 63, 7, 14, 17>;
+SPX_REGISTER_SET(cuspx_set_sha2_192s, "sha2-192s", 2, SetParams, 6)

@@ -1,0 +1,5 @@
+// shake256-128s: n=16 h=63 d=7 a=12 k=14 (params.h:33-79); FORS: 2^4 leaves per lane
+#include "set_impl.hpp"
+
+using SetParams = spx::Params<spx::kShake, 16, 63, 7, 12, 14>;
+SPX_REGISTER_SET(cuspx_set_shake256_128s, "shake256-128s", 6, SetParams, 4)

@@ -1,0 +1,5 @@
+// shake256-192f: n=24 h=66 d=22 a=8 k=33 (params.h:33-79); FORS: 2^2 leaves per lane
+#include "set_impl.hpp"
+
+using SetParams = spx::Params<spx::kShake, 24, 66, 22, 8, 33>;
+SPX_REGISTER_SET(cuspx_set_shake256_192f, "shake256-192f", 9, SetParams, 2)

@@ -1,0 +1,323 @@
+// sha2_dev.hpp -- SHA-256 tweakable-hash family for CDNA4 (gfx950).
+//
+// Semantics follow the reference host path:
+//   thash  T_l(x) = SHA256(mid(pub_seed) ; addr[0..22) || x)[:n]   thash_sha256.cu:16-34
+//   PRF(a)        = SHA256(sk_seed || addr[0..22))[:n]             hash_sha256.cu:65-76
+//   PRF_msg       = HMAC-SHA256(sk_prf, optrand || m)[:n]          hash_sha256.cu:99-144
+//   H_msg         = MGF1(SHA256(R || pk || m), dgst)               hash_sha256.cu:200-262
+// where mid(pub_seed) is the state after the single block pub_seed||0^(64-n)
+// (seed_state, sha256.cu:391-405) -- computed once per key and carried in
+// the per-request plan, never shared between keys.
+//
+// Values are big-endian words.  The 22-byte address puts every payload at a
+// 2-byte offset inside the block, so payload words are re-aligned with
+// v_alignbit (one op per word).  The compression is fully unrolled with
+// v_alignbit rotates and v_bitop3 for Ch / Maj / 3-way xor: 14 VALU ops per
+// round and 10 per schedule word.
+#pragma once
+#include "spx_common.hpp"
+
+namespace spx {
+
+static constexpr uint32_t kSha256K[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+static constexpr uint32_t kSha256IV[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                                          0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+
+// One FIPS 180-4 compression.  w[] is used as the rolling schedule.
+__device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) {
+    uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
+    uint32_t e = st[4], f = st[5], g = st[6], h = st[7];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        uint32_t wi;
+        if (i < 16) {
+            wi = w[i];
+        } else {
+            uint32_t x = w[(i - 15) & 15], y = w[(i - 2) & 15];
+            uint32_t s0 = xor3(rotr32(x, 7), rotr32(x, 18), x >> 3);
+            uint32_t s1 = xor3(rotr32(y, 17), rotr32(y, 19), y >> 10);
+            wi = s1 + w[(i - 7) & 15] + s0 + w[i & 15];
+            w[i & 15] = wi;
+        }
+        uint32_t t1 = h + xor3(rotr32(e, 6), rotr32(e, 11), rotr32(e, 25)) + bit_ch(e, f, g) +
+                      kSha256K[i] + wi;
+        uint32_t t2 = xor3(rotr32(a, 2), rotr32(a, 13), rotr32(a, 22)) + bit_maj(a, b, c);
+        h = g;
+        g = f;
+        f = e;
+        e = d + t1;
+        d = c;
+        c = b;
+        b = a;
+        a = t1 + t2;
+    }
+    st[0] += a;
+    st[1] += b;
+    st[2] += c;
+    st[3] += d;
+    st[4] += e;
+    st[5] += f;
+    st[6] += g;
+    st[7] += h;
+}
+
+// write x into b[q] for a wave-uniform runtime q (v_cndmask per slot)
+__device__ __forceinline__ void put16(uint32_t b[16], int q, uint32_t x) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) b[i] = (q == i) ? x : b[i];
+}
+
+template <int N>
+struct Sha2 {
+    static constexpr int hash = kSha2;
+    static constexpr int NW = N / 4;
+
+    struct Key {
+        uint32_t mid[8];
+        uint32_t sk[NW];
+    };
+
+    __device__ static __forceinline__ void load_key(const Plan& p, Key& k) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) k.mid[i] = p.mid[i];
+#pragma unroll
+        for (int i = 0; i < NW; i++) k.sk[i] = p.skseed[i];
+    }
+
+    // T over M payload words after the 22-byte address, all block indices
+    // known at compile time (used for F, H and short T_l).
+    template <int M>
+    __device__ static __forceinline__ void thash_words(const Key& k, const Adr& a,
+                                                       const uint32_t d[M], uint32_t out[NW]) {
+        constexpr int P = 6 + M;                      // words up to and incl. the 0x80 byte
+        constexpr int B = (P + 2 + 15) / 16;          // compression blocks
+        constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+        uint32_t st[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = k.mid[i];
+#pragma unroll
+        for (int blk = 0; blk < B; blk++) {
+            uint32_t w[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int g = 16 * blk + i;
+                uint32_t x;
+                if (g < 5) x = a.w[g];
+                else if (g == 5) x = a.w[5] | (d[0] >> 16);
+                else if (g < 5 + M) x = __builtin_amdgcn_alignbit(d[g - 6], d[g - 5], 16);
+                else if (g == 5 + M) x = (d[M - 1] << 16) | 0x8000u;
+                else if (g == 16 * B - 1) x = bits;
+                else x = 0;
+                w[i] = x;
+            }
+            sha256_compress(st, w);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; i++) out[i] = st[i];
+    }
+
+    // F: one input value (wots.cu gen_chain step, fors leaf)
+    __device__ static __forceinline__ void F(const Key& k, const Adr& a, const uint32_t in[NW],
+                                             uint32_t out[NW]) {
+        thash_words<NW>(k, a, in, out);
+    }
+
+    // H: two input values (Merkle node)
+    __device__ static __forceinline__ void H(const Key& k, const Adr& a, const uint32_t l[NW],
+                                             const uint32_t r[NW], uint32_t out[NW]) {
+        uint32_t d[2 * NW];
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            d[i] = l[i];
+            d[NW + i] = r[i];
+        }
+        thash_words<2 * NW>(k, a, d, out);
+    }
+
+    // PRF(sk_seed, addr) from the IV: sk_seed || addr22 || pad, one block
+    __device__ static __forceinline__ void PRF(const Key& k, const Adr& a, uint32_t out[NW]) {
+        uint32_t w[16], st[8];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            uint32_t x;
+            if (i < NW) x = k.sk[i];
+            else if (i < NW + 5) x = a.w[i - NW];
+            else if (i == NW + 5) x = a.w[5] | 0x8000u;
+            else if (i == 15) x = (uint32_t)(N + 22) * 8u;
+            else x = 0;
+            w[i] = x;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
+        sha256_compress(st, w);
+#pragma unroll
+        for (int i = 0; i < NW; i++) out[i] = st[i];
+    }
+
+    // Streaming T_l: items of NW words appended one word at a time with a
+    // wave-uniform position; used for T_len (leaf) and T_k (FORS pk).
+    struct Stream {
+        uint32_t st[8];
+        uint32_t b[16];
+        int q;
+        uint32_t carry;
+
+        __device__ __forceinline__ void begin(const Key& k, const Adr& a) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) st[i] = k.mid[i];
+#pragma unroll
+            for (int i = 0; i < 16; i++) b[i] = i < 5 ? a.w[i] : 0u;
+            q = 5;
+            carry = a.w[5] >> 16;
+        }
+        // append one payload word (must be called from a single site in a loop)
+        __device__ __forceinline__ void word(uint32_t dw) {
+            put16(b, q, __builtin_amdgcn_alignbit(carry, dw, 16));
+            carry = dw;
+            q++;
+            if (q == 16) {
+                sha256_compress(st, b);
+                q = 0;
+            }
+        }
+        // D = total payload words appended; finishes with compile-time layout
+        template <int D>
+        __device__ __forceinline__ void finish(uint32_t out[NW]) {
+            constexpr int qe = (5 + D) % 16;
+            constexpr uint32_t bits = (64u + 22u + 4u * D) * 8u;
+            b[qe] = (carry << 16) | 0x8000u;
+#pragma unroll
+            for (int i = qe + 1; i < 16; i++) b[i] = 0;
+            if (qe <= 13) {
+                b[15] = bits;
+                sha256_compress(st, b);
+            } else {
+                sha256_compress(st, b);
+#pragma unroll
+                for (int i = 0; i < 15; i++) b[i] = 0;
+                b[15] = bits;
+                sha256_compress(st, b);
+            }
+#pragma unroll
+            for (int i = 0; i < NW; i++) out[i] = st[i];
+        }
+    };
+
+    // ------------------------------------------------ per-request hashing
+
+    // midstate: compress(IV, pub_seed || 0^(64-n)) (seed_state, sha256.cu:391)
+    __device__ static __forceinline__ void midstate(const uint32_t pub[NW], uint32_t mid[8]) {
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = i < NW ? pub[i] : 0u;
+#pragma unroll
+        for (int i = 0; i < 8; i++) mid[i] = kSha256IV[i];
+        sha256_compress(mid, w);
+    }
+
+    // SHA-256 continuing from st over (pre[0..NPRE) || m[0..mlen)), where
+    // `before` bytes were already absorbed.  m is 4-byte aligned; bytes past
+    // mlen are never read beyond the word holding the last byte.
+    template <int NPRE>
+    __device__ static void stream_msg(uint32_t st[8], const uint32_t pre[NPRE],
+                                      const uint8_t* m, uint32_t mlen, uint32_t before) {
+        const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
+        const uint32_t ndata = NPRE + (mlen + 4) / 4;  // words holding data or 0x80
+        const uint32_t total = ((ndata + 2 + 15) / 16) * 16;
+        const uint32_t* mw = reinterpret_cast<const uint32_t*>(m);
+        uint32_t b[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) b[i] = 0;
+        for (uint32_t kk = 0; kk < total; kk++) {
+            uint32_t x;
+            if (kk < (uint32_t)NPRE) {
+                x = pre[0];
+#pragma unroll
+                for (int j = 1; j < NPRE; j++) x = (kk == (uint32_t)j) ? pre[j] : x;
+            } else if (kk < ndata) {
+                uint32_t i = kk - NPRE;
+                int valid = (int)mlen - 4 * (int)i;
+                uint32_t raw = valid > 0 ? bswap32(mw[i]) : 0u;
+                if (valid >= 4) x = raw;
+                else {
+                    uint32_t keep = valid <= 0 ? 0u : (0xFFFFFFFFu << (8 * (4 - valid)));
+                    x = (raw & keep) | (0x80000000u >> (8 * (valid < 0 ? 0 : valid)));
+                }
+            } else if (kk == total - 2) {
+                x = (uint32_t)(bits >> 32);
+            } else if (kk == total - 1) {
+                x = (uint32_t)bits;
+            } else {
+                x = 0;
+            }
+            put16(b, (int)(kk & 15), x);
+            if ((kk & 15) == 15) sha256_compress(st, b);
+        }
+    }
+
+    // R = HMAC-SHA256(sk_prf, optrand || m)[:n]  (hash_sha256.cu:99-144)
+    __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
+                                   const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
+        uint32_t st[8], w[16], inner[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x36363636u;
+        sha256_compress(st, w);
+        stream_msg<NW>(st, opt, m, mlen, 64);
+#pragma unroll
+        for (int i = 0; i < 8; i++) inner[i] = st[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x5c5c5c5cu;
+        sha256_compress(st, w);
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            w[i] = i < 8 ? inner[i] : (i == 8 ? 0x80000000u : (i == 15 ? 768u : 0u));
+        sha256_compress(st, w);
+#pragma unroll
+        for (int i = 0; i < NW; i++) R[i] = st[i];
+    }
+
+    // digest bytes = MGF1(SHA256(R || pk || m), DG)  (hash_sha256.cu:200-247)
+    template <int DG>
+    __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
+                                 const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
+        uint32_t pre[3 * NW], st[8], seed[8], w[16];
+#pragma unroll
+        for (int i = 0; i < NW; i++) pre[i] = R[i];
+#pragma unroll
+        for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
+        stream_msg<3 * NW>(st, pre, m, mlen, 0);
+#pragma unroll
+        for (int i = 0; i < 8; i++) seed[i] = st[i];
+#pragma unroll
+        for (int c = 0; c < (DG + 31) / 32; c++) {  // mgf1, sha256.cu:362-383
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                w[i] = i < 8 ? seed[i]
+                             : (i == 8 ? (uint32_t)c
+                                       : (i == 9 ? 0x80000000u : (i == 15 ? 288u : 0u)));
+#pragma unroll
+            for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
+            sha256_compress(st, w);
+#pragma unroll
+            for (int i = 0; i < 32; i++)
+                if (32 * c + i < DG) dg[32 * c + i] = (uint8_t)(st[i >> 2] >> (24 - 8 * (i & 3)));
+        }
+    }
+};
+
+}  // namespace spx

@@ -1,0 +1,309 @@
+// shake_dev.hpp -- SHAKE256 tweakable-hash family for CDNA4 (gfx950).
+//
+// Semantics follow the reference host path:
+//   thash  T_l(x) = SHAKE256(pub_seed || addr[0..32) || x, n)   thash_shake256.cu:15-25
+//   PRF(a)        = SHAKE256(sk_seed || addr[0..32), n)         hash_shake256.cu:38-47
+//   PRF_msg       = SHAKE256(sk_prf || optrand || m, n)         hash_shake256.cu:64-75
+//   H_msg         = SHAKE256(R || pk || m, dgst)                hash_shake256.cu:96-130
+// The address uses the SHA-256 byte layout (params.h:148), so bytes 22..31
+// are zero.
+//
+// Keccak-f[1600] runs on 32-bit lane halves: the 200-byte state is 50
+// little-endian words s[2i] (low) / s[2i+1] (high), so sponge byte j sits in
+// word j/4.  64-bit rotations are two v_alignbit, theta's column parity two
+// v_bitop3 (3-way xor) per half, theta application one 3-way xor per word and
+// chi one v_bitop3 (a ^ (~b & c)) per word.
+#pragma once
+#include "spx_common.hpp"
+
+namespace spx {
+
+__constant__ uint32_t kKeccakRC[48] = {
+    0x00000001u, 0x00000000u, 0x00008082u, 0x00000000u, 0x0000808au, 0x80000000u,
+    0x80008000u, 0x80000000u, 0x0000808bu, 0x00000000u, 0x80000001u, 0x00000000u,
+    0x80008081u, 0x80000000u, 0x00008009u, 0x80000000u, 0x0000008au, 0x00000000u,
+    0x00000088u, 0x00000000u, 0x80008009u, 0x00000000u, 0x8000000au, 0x00000000u,
+    0x8000808bu, 0x00000000u, 0x0000008bu, 0x80000000u, 0x00008089u, 0x80000000u,
+    0x00008003u, 0x80000000u, 0x00008002u, 0x80000000u, 0x00000080u, 0x80000000u,
+    0x0000800au, 0x00000000u, 0x8000000au, 0x80000000u, 0x80008081u, 0x80000000u,
+    0x00008080u, 0x80000000u, 0x80000001u, 0x00000000u, 0x80008008u, 0x80000000u};
+
+// rotation offsets r[x + 5y]
+static constexpr int kRho[25] = {0,  1,  62, 28, 27, 36, 44, 6,  55, 20, 3,  10, 43,
+                                 25, 39, 41, 45, 15, 21, 8,  18, 2,  61, 56, 14};
+
+template <int R>
+__device__ __forceinline__ void rol64(uint32_t lo, uint32_t hi, uint32_t& olo, uint32_t& ohi) {
+    if (R == 0) {
+        olo = lo;
+        ohi = hi;
+    } else if (R < 32) {
+        ohi = __builtin_amdgcn_alignbit(hi, lo, 32 - R);
+        olo = __builtin_amdgcn_alignbit(lo, hi, 32 - R);
+    } else if (R == 32) {
+        olo = hi;
+        ohi = lo;
+    } else {
+        ohi = __builtin_amdgcn_alignbit(lo, hi, 64 - R);
+        olo = __builtin_amdgcn_alignbit(hi, lo, 64 - R);
+    }
+}
+
+// one round of Keccak-f[1600] on s[50] with round constant (rc_lo, rc_hi)
+__device__ __forceinline__ void keccak_round(uint32_t s[50], uint32_t rc_lo, uint32_t rc_hi) {
+    uint32_t C[10];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            C[2 * x + h] = xor3(xor3(s[2 * x + h], s[2 * (x + 5) + h], s[2 * (x + 10) + h]),
+                                s[2 * (x + 15) + h], s[2 * (x + 20) + h]);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        uint32_t rlo, rhi;
+        const int xp = (x + 1) % 5, xm = (x + 4) % 5;
+        rol64<1>(C[2 * xp], C[2 * xp + 1], rlo, rhi);
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            s[2 * (x + 5 * y)] = xor3(s[2 * (x + 5 * y)], C[2 * xm], rlo);
+            s[2 * (x + 5 * y) + 1] = xor3(s[2 * (x + 5 * y) + 1], C[2 * xm + 1], rhi);
+        }
+    }
+    uint32_t B[50];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            // pi: B[y, 2x + 3y] = rol(A[x, y], r[x, y])
+            const int src = x + 5 * y;
+            const int dst = y + 5 * ((2 * x + 3 * y) % 5);
+            uint32_t lo, hi;
+            switch (kRho[src]) {
+#define SPX_ROT_CASE(R) \
+    case R: rol64<R>(s[2 * src], s[2 * src + 1], lo, hi); break;
+                SPX_ROT_CASE(0) SPX_ROT_CASE(1) SPX_ROT_CASE(62) SPX_ROT_CASE(28)
+                SPX_ROT_CASE(27) SPX_ROT_CASE(36) SPX_ROT_CASE(44) SPX_ROT_CASE(6)
+                SPX_ROT_CASE(55) SPX_ROT_CASE(20) SPX_ROT_CASE(3) SPX_ROT_CASE(10)
+                SPX_ROT_CASE(43) SPX_ROT_CASE(25) SPX_ROT_CASE(39) SPX_ROT_CASE(41)
+                SPX_ROT_CASE(45) SPX_ROT_CASE(15) SPX_ROT_CASE(21) SPX_ROT_CASE(8)
+                SPX_ROT_CASE(18) SPX_ROT_CASE(2) SPX_ROT_CASE(61) SPX_ROT_CASE(56)
+                SPX_ROT_CASE(14)
+#undef SPX_ROT_CASE
+                default: lo = s[2 * src]; hi = s[2 * src + 1]; break;
+            }
+            B[2 * dst] = lo;
+            B[2 * dst + 1] = hi;
+        }
+    }
+#pragma unroll
+    for (int y = 0; y < 5; y++) {
+#pragma unroll
+        for (int x = 0; x < 5; x++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                s[2 * (x + 5 * y) + h] = bit_chi(B[2 * (x + 5 * y) + h],
+                                                 B[2 * ((x + 1) % 5 + 5 * y) + h],
+                                                 B[2 * ((x + 2) % 5 + 5 * y) + h]);
+            }
+        }
+    }
+    s[0] ^= rc_lo;
+    s[1] ^= rc_hi;
+}
+
+// Keccak-f[1600]: round 0 unrolled (so constant state words fold after an
+// absorb), rounds 1..23 in a loop with the constants from constant memory.
+__device__ __forceinline__ void keccak_f(uint32_t s[50]) {
+    keccak_round(s, 0x00000001u, 0x00000000u);
+#pragma unroll 1
+    for (int r = 1; r < 24; r++) keccak_round(s, kKeccakRC[2 * r], kKeccakRC[2 * r + 1]);
+}
+
+// xor x into s[q] for a wave-uniform runtime q < 34 (the rate)
+__device__ __forceinline__ void xor_at(uint32_t s[50], int q, uint32_t x) {
+#pragma unroll
+    for (int i = 0; i < 34; i++) s[i] ^= (q == i) ? x : 0u;
+}
+
+template <int N>
+struct Shake {
+    static constexpr int hash = kShake;
+    static constexpr int NW = N / 4;
+
+    struct Key {
+        uint32_t pub[NW];
+        uint32_t sk[NW];
+    };
+
+    __device__ static __forceinline__ void load_key(const Plan& p, Key& k) {
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            k.pub[i] = p.pubseed[i];
+            k.sk[i] = p.skseed[i];
+        }
+    }
+
+    // address byte-words (big-endian) as sponge words (little-endian)
+    __device__ static __forceinline__ uint32_t aw(const Adr& a, int i) {
+        return i < 6 ? bswap32(a.w[i]) : 0u;
+    }
+
+    // single-permutation SHAKE256 over pre(NW) || addr32 || d(M), output n
+    template <int M>
+    __device__ static __forceinline__ void one_block(const uint32_t pre[NW], const Adr& a,
+                                                     const uint32_t* d, uint32_t out[NW]) {
+        constexpr int L = NW + 8 + M;  // input words
+        static_assert(4 * L <= 135, "input must fit one block");
+        uint32_t s[50];
+#pragma unroll
+        for (int i = 0; i < 50; i++) {
+            uint32_t x = 0;
+            if (i < NW) x = pre[i];
+            else if (i < NW + 8) x = aw(a, i - NW);
+            else if (i < L) x = d[i - NW - 8];
+            if (i == L) x ^= 0x1Fu;
+            if (i == 33) x ^= 0x80000000u;
+            s[i] = x;
+        }
+        keccak_f(s);
+#pragma unroll
+        for (int i = 0; i < NW; i++) out[i] = s[i];
+    }
+
+    __device__ static __forceinline__ void F(const Key& k, const Adr& a, const uint32_t in[NW],
+                                             uint32_t out[NW]) {
+        one_block<NW>(k.pub, a, in, out);
+    }
+
+    __device__ static __forceinline__ void H(const Key& k, const Adr& a, const uint32_t l[NW],
+                                             const uint32_t r[NW], uint32_t out[NW]) {
+        uint32_t d[2 * NW];
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            d[i] = l[i];
+            d[NW + i] = r[i];
+        }
+        one_block<2 * NW>(k.pub, a, d, out);
+    }
+
+    __device__ static __forceinline__ void PRF(const Key& k, const Adr& a, uint32_t out[NW]) {
+        one_block<0>(k.sk, a, nullptr, out);
+    }
+
+    // Streaming T_l.  Items are parked in a per-lane global buffer (layout
+    // [word][lane] for coalescing) and absorbed in one pass by finish(), so
+    // the 50-word sponge is not live across the WOTS chain loop.
+    struct Stream {
+        const Key* key;
+        Adr adr;
+        uint32_t* buf;
+        uint32_t stride;  // lanes in the buffer
+
+        __device__ __forceinline__ void begin(const Key& k, const Adr& a, uint32_t* lane_buf,
+                                              uint32_t lane_stride) {
+            key = &k;
+            adr = a;
+            buf = lane_buf;
+            stride = lane_stride;
+        }
+        __device__ __forceinline__ void item(int j, const uint32_t v[NW]) {
+#pragma unroll
+            for (int i = 0; i < NW; i++) buf[(size_t)(j * NW + i) * stride] = v[i];
+        }
+        template <int ITEMS>
+        __device__ __forceinline__ void finish(uint32_t out[NW]) {
+            constexpr int D = ITEMS * NW;            // data words
+            constexpr int T = NW + 8 + D;            // absorbed words
+            constexpr int NB = T / 34 + 1;           // permutations (pad always fits)
+            constexpr int TAIL = T - 34 * (NB - 1);  // words in the last block
+            uint32_t s[50];
+#pragma unroll
+            for (int i = 0; i < 50; i++) s[i] = 0;
+#pragma unroll
+            for (int i = 0; i < NW + 8; i++) s[i] = i < NW ? key->pub[i] : aw(adr, i - NW);
+            int q = NW + 8;
+#pragma unroll 1
+            for (int u = 0; u < D; u++) {
+                xor_at(s, q, buf[(size_t)u * stride]);
+                q++;
+                if (q == 34) {
+                    keccak_f(s);
+                    q = 0;
+                }
+            }
+            s[TAIL] ^= 0x1Fu;
+            s[33] ^= 0x80000000u;
+            keccak_f(s);
+#pragma unroll
+            for (int i = 0; i < NW; i++) out[i] = s[i];
+        }
+    };
+
+    // ------------------------------------------------ per-request hashing
+
+    // SHAKE256(pre || m) squeezed to OUTB bytes (OUTB <= 136).  m is 4-byte
+    // aligned; the word holding the last message byte is the last one read.
+    template <int NPRE, int OUTB>
+    __device__ static void stream_msg(const uint32_t pre[NPRE], const uint8_t* m, uint32_t mlen,
+                                      uint8_t out[OUTB]) {
+        static_assert(NPRE < 34 && OUTB <= 136, "prefix within one block");
+        const uint32_t* mw = reinterpret_cast<const uint32_t*>(m);
+        uint32_t s[50];
+#pragma unroll
+        for (int i = 0; i < 50; i++) s[i] = i < NPRE ? pre[i] : 0u;
+        int q = NPRE;
+        const uint32_t full = mlen / 4, rem = mlen & 3;
+#pragma unroll 1
+        for (uint32_t i = 0; i < full; i++) {
+            xor_at(s, q, mw[i]);
+            q++;
+            if (q == 34) {
+                keccak_f(s);
+                q = 0;
+            }
+        }
+        uint32_t last = rem ? (mw[full] & (0xFFFFFFFFu >> (8 * (4 - rem)))) : 0u;
+        last |= 0x1Fu << (8 * rem);
+        xor_at(s, q, last);
+        s[33] ^= 0x80000000u;
+        keccak_f(s);
+#pragma unroll
+        for (int i = 0; i < OUTB; i++) out[i] = (uint8_t)(s[i >> 2] >> (8 * (i & 3)));
+    }
+
+    __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
+                                   const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
+        uint32_t pre[2 * NW];
+        uint8_t o[N];
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            pre[i] = skprf[i];
+            pre[NW + i] = opt[i];
+        }
+        stream_msg<2 * NW, N>(pre, m, mlen, o);
+#pragma unroll
+        for (int i = 0; i < NW; i++)
+            R[i] = (uint32_t)o[4 * i] | (uint32_t)o[4 * i + 1] << 8 |
+                   (uint32_t)o[4 * i + 2] << 16 | (uint32_t)o[4 * i + 3] << 24;
+    }
+
+    template <int DG>
+    __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
+                                 const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
+        uint32_t pre[3 * NW];
+#pragma unroll
+        for (int i = 0; i < NW; i++) pre[i] = R[i];
+#pragma unroll
+        for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
+        stream_msg<3 * NW, DG>(pre, m, mlen, dg);
+    }
+
+    __device__ static __forceinline__ void midstate(const uint32_t*, uint32_t mid[8]) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) mid[i] = 0;
+    }
+};
+
+}  // namespace spx

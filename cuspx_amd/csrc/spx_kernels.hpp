@@ -1,0 +1,635 @@
+// spx_kernels.hpp -- batched SPHINCS+ sign / verify / keygen kernels.
+//
+// One signature is split across many lanes (MI355X needs >> 256 workgroups
+// and prefers uniform control flow inside a wave):
+//
+//   sign:   k_prep       lane per request: midstate, PRF_msg, H_msg, plan
+//           k_fors       L lanes per FORS tree (2^J leaves per lane), LDS reduce
+//           k_fors_pk    lane per request: T_k over the k roots
+//           k_ht         lane per WOTS leaf of every hypertree layer (all d
+//                        layers at once: every layer's tree/leaf comes from
+//                        H_msg alone), LDS reduce, auth path, layer root;
+//                        the signing leaf's chain values are captured
+//           k_wots_gather lane per (request, layer, chain): picks the
+//                        captured chain value for the layer's message
+//   verify: k_vprep, k_vfors, k_fors_pk, k_vht (len lanes per request, layers
+//           in sequence)
+//   keygen: k_kprep, k_ht in keygen mode (top layer only)
+//
+// Semantics follow sign.cu:1924-1994 (sign), sign.cu:3048-3121 (verify),
+// sign.cu:154-182 (keygen), utils.cu:94-241 (compute_root / treehash),
+// wots.cu:27-790 and fors.cu:24-633 of the reference.
+#pragma once
+#include <type_traits>
+
+#include "sha2_dev.hpp"
+#include "shake_dev.hpp"
+
+namespace spx {
+
+template <class P>
+using HashT = typename std::conditional<P::hash == kSha2, Sha2<P::n>, Shake<P::n>>::type;
+
+// ------------------------------------------------------------- arguments
+
+struct BatchArgs {
+    // inputs
+    const uint8_t* sk;        // sign: secret keys, stride sk_stride
+    uint32_t sk_stride;
+    const uint8_t* pk;        // verify: public keys, stride pk_stride
+    uint32_t pk_stride;
+    const uint8_t* seed;      // keygen: 3n-byte seeds, stride seed_stride
+    uint32_t seed_stride;
+    const uint8_t* msg;       // messages: msg + moff[i], mlen[i] bytes, 4-byte aligned
+    const uint64_t* moff;
+    const uint32_t* mlen;
+    const uint8_t* optrand;   // sign: [N][n]
+    // outputs
+    uint8_t* sig;             // signatures (sign: written, verify: read)
+    uint64_t sig_stride;
+    int32_t* status;          // verify: 0 ok / -1 fail per request
+    uint8_t* pk_out;          // keygen outputs
+    uint8_t* sk_out;
+    // workspace
+    Plan* plan;
+    uint8_t* fors_roots;      // [N][k][n]
+    uint8_t* fors_pk;         // [N][n]
+    uint8_t* roots;           // [N][d][n]
+    uint8_t* cap;             // [N][d][len][16][n]
+    uint32_t* tl_buf;         // SHAKE T_len parking: [len*nw][lanes]
+    uint32_t tl_lanes;
+    uint32_t n_req;
+};
+
+// ----------------------------------------------------- message / indices
+
+template <class P>
+__device__ __forceinline__ void plan_from_digest(Plan& pl, const uint8_t* dg) {
+    // message_to_indices (fors.cu:77-91): LSB-first bits within each byte
+#pragma unroll
+    for (int t = 0; t < P::k; t++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = 0; j < P::a; j++) {
+            const int off = t * P::a + j;
+            v |= (uint32_t)((dg[off >> 3] >> (off & 7)) & 1) << j;
+        }
+        pl.fors_idx[t] = v;
+    }
+    // tree / leaf (hash_sha256.cu:249-261): big-endian, masked
+    uint64_t tree = 0;
+#pragma unroll
+    for (int i = 0; i < P::tree_bytes; i++)
+        tree = (tree << 8) | dg[P::fors_msg_bytes + i];
+    if (P::tree_bits < 64) tree &= (~0ull) >> (64 - P::tree_bits);
+    uint32_t leaf = 0;
+#pragma unroll
+    for (int i = 0; i < P::leaf_bytes; i++)
+        leaf = (leaf << 8) | dg[P::fors_msg_bytes + P::tree_bytes + i];
+    leaf &= (~0u) >> (32 - P::leaf_bits);
+    pl.tree = tree;
+    pl.leaf = leaf;
+}
+
+// sign: midstate, R = PRF_msg, digest = H_msg, plan
+template <class P>
+__global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
+    using Hh = HashT<P>;
+    constexpr int NW = P::nw;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_req) return;
+    const uint8_t* sk = A.sk + (size_t)i * A.sk_stride;
+    uint32_t skseed[NW], skprf[NW], pub[NW], pk[2 * NW], opt[NW], R[NW];
+    load_val<P::hash, NW>(sk, skseed);
+    load_val<P::hash, NW>(sk + P::n, skprf);
+    load_val<P::hash, NW>(sk + 2 * P::n, pub);
+    load_val<P::hash, 2 * NW>(sk + 2 * P::n, pk);
+    load_val<P::hash, NW>(A.optrand + (size_t)i * P::n, opt);
+    const uint8_t* m = A.msg + A.moff[i];
+    const uint32_t ml = A.mlen[i];
+    Plan pl;
+    Hh::midstate(pub, pl.mid);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        pl.skseed[j] = j < NW ? skseed[j] : 0u;
+        pl.pubseed[j] = j < NW ? pub[j] : 0u;
+        pl.root[j] = 0;
+    }
+    Hh::prf_msg(skprf, opt, m, ml, R);
+    uint8_t* sig = A.sig + (size_t)i * A.sig_stride;
+    store_val<P::hash, NW>(sig, R);
+    uint8_t dg[P::dgst_bytes];
+    Hh::template h_msg<P::dgst_bytes>(R, pk, m, ml, dg);
+    plan_from_digest<P>(pl, dg);
+    pl.status = 0;
+    A.plan[i] = pl;
+}
+
+// verify: midstate, digest = H_msg(R from the signature), plan
+template <class P>
+__global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
+    using Hh = HashT<P>;
+    constexpr int NW = P::nw;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_req) return;
+    const uint8_t* pkp = A.pk + (size_t)i * A.pk_stride;
+    uint32_t pub[NW], pk[2 * NW], R[NW];
+    load_val<P::hash, NW>(pkp, pub);
+    load_val<P::hash, 2 * NW>(pkp, pk);
+    const uint8_t* sig = A.sig + (size_t)i * A.sig_stride;
+    load_val<P::hash, NW>(sig, R);
+    Plan pl;
+    Hh::midstate(pub, pl.mid);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        pl.skseed[j] = 0;
+        pl.pubseed[j] = j < NW ? pub[j] : 0u;
+        pl.root[j] = j < NW ? pk[NW + j] : 0u;
+    }
+    uint8_t dg[P::dgst_bytes];
+    Hh::template h_msg<P::dgst_bytes>(R, pk, A.msg + A.moff[i], A.mlen[i], dg);
+    plan_from_digest<P>(pl, dg);
+    pl.status = 0;
+    A.plan[i] = pl;
+}
+
+// keygen: sk = seed (3n) || root, plan with the key material
+template <class P>
+__global__ void __launch_bounds__(64) k_kprep(BatchArgs A) {
+    using Hh = HashT<P>;
+    constexpr int NW = P::nw;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_req) return;
+    const uint8_t* s = A.seed + (size_t)i * A.seed_stride;
+    uint32_t skseed[NW], pub[NW];
+    load_val<P::hash, NW>(s, skseed);
+    load_val<P::hash, NW>(s + 2 * P::n, pub);
+    Plan pl;
+    Hh::midstate(pub, pl.mid);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        pl.skseed[j] = j < NW ? skseed[j] : 0u;
+        pl.pubseed[j] = j < NW ? pub[j] : 0u;
+        pl.root[j] = 0;
+    }
+    pl.tree = 0;
+    pl.leaf = 0;
+    pl.status = 0;
+    A.plan[i] = pl;
+    // sk = SK_SEED || SK_PRF || PUB_SEED (|| root, written by k_ht); pk = PUB_SEED || root
+    uint8_t* sko = A.sk_out + (size_t)i * P::sk_bytes;
+    uint8_t* pko = A.pk_out + (size_t)i * P::pk_bytes;
+    for (int j = 0; j < 3 * P::n; j++) sko[j] = s[j];
+    for (int j = 0; j < P::n; j++) pko[j] = s[2 * P::n + j];
+}
+
+// ------------------------------------------------------------------ FORS
+
+// sign: FORS trees, L = 2^(a-J) lanes per tree, 2^J leaves per lane
+template <class P, int J>
+__global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr int NW = P::nw;
+    constexpr int LOGL = P::a - J;
+    constexpr int L = 1 << LOGL;
+    constexpr int WG = 256;
+    static_assert(L <= WG, "tree wider than a workgroup");
+    constexpr int TPW = WG / L;
+    __shared__ uint32_t lds[2][WG][NW];
+
+    const int tid = threadIdx.x;
+    const uint32_t g = blockIdx.x * TPW + tid / L;  // global tree id
+    const uint32_t u = tid % L;
+    const bool active = g < A.n_req * P::k;
+    const uint32_t req = active ? g / P::k : 0;
+    const uint32_t t = active ? g % P::k : 0;
+    const Plan& pl = A.plan[req];
+    Key key;
+    Hh::load_key(pl, key);
+    const uint32_t target = pl.fors_idx[t];
+    Adr adr = adr_make(0, pl.tree, kForsTree);
+    adr_keypair<P::th>(adr, pl.leaf);
+    uint8_t* rec = A.sig + (size_t)req * A.sig_stride + P::n + (size_t)t * (P::a + 1) * P::n;
+
+    uint32_t node[NW];
+    uint32_t stk[J > 0 ? J : 1][NW];
+    const uint32_t base = (t << P::a) + (u << J);  // global leaf index of this lane's first leaf
+#pragma unroll 1
+    for (uint32_t i = 0; i < (1u << J); i++) {
+        const uint32_t li = (u << J) + i;  // leaf index within the tree
+        uint32_t sk[NW];
+        adr_node(adr, 0, base + i);
+        Hh::PRF(key, adr, sk);
+        Hh::F(key, adr, sk, node);
+        if (active && li == target) store_val<P::hash, NW>(rec, sk);
+        if (active && li == (target ^ 1u)) store_val<P::hash, NW>(rec + P::n, node);
+        // merge with pending left siblings (i is uniform across the wave)
+        uint32_t h = 0;
+#pragma unroll 1
+        while (h < (uint32_t)J && ((i >> h) & 1u)) {
+            uint32_t left[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                uint32_t x = stk[0][w];
+#pragma unroll
+                for (int s = 1; s < J; s++) x = (h == (uint32_t)s) ? stk[s][w] : x;
+                left[w] = x;
+            }
+            h++;
+            const uint32_t idx = li >> h;  // node index at height h within the tree
+            adr_node(adr, h, ((t << P::a) >> h) + idx);
+            Hh::H(key, adr, left, node, node);
+            if (active && idx == ((target >> h) ^ 1u))
+                store_val<P::hash, NW>(rec + P::n + (size_t)h * P::n, node);
+        }
+        if (h < (uint32_t)J) {
+#pragma unroll
+            for (int s = 0; s < J; s++)
+#pragma unroll
+                for (int w = 0; w < NW; w++) stk[s][w] = (h == (uint32_t)s) ? node[w] : stk[s][w];
+        }
+    }
+    // cross-lane reduction: level l holds WG >> l nodes, node x = H(2x, 2x+1)
+#pragma unroll
+    for (int w = 0; w < NW; w++) lds[0][tid][w] = node[w];
+#pragma unroll
+    for (int l = 1; l <= LOGL; l++) {
+        __syncthreads();
+        const int src = (l - 1) & 1, dst = l & 1;
+        if (tid < (WG >> l)) {
+            const uint32_t tau = tid >> (LOGL - l);
+            const uint32_t x = tid & ((L >> l) - 1);
+            const uint32_t g2 = blockIdx.x * TPW + tau;
+            const bool act2 = g2 < A.n_req * P::k;
+            const uint32_t req2 = act2 ? g2 / P::k : 0, t2 = act2 ? g2 % P::k : 0;
+            const Plan& p2 = A.plan[req2];
+            Key k2;
+            Hh::load_key(p2, k2);
+            uint32_t lft[NW], rgt[NW], out[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                lft[w] = lds[src][2 * tid][w];
+                rgt[w] = lds[src][2 * tid + 1][w];
+            }
+            const uint32_t hgt = J + l;
+            Adr a2 = adr_make(0, p2.tree, kForsTree);
+            adr_keypair<P::th>(a2, p2.leaf);
+            adr_node(a2, hgt, ((t2 << P::a) >> hgt) + x);
+            Hh::H(k2, a2, lft, rgt, out);
+#pragma unroll
+            for (int w = 0; w < NW; w++) lds[dst][tid][w] = out[w];
+            if (act2) {
+                uint8_t* rec2 =
+                    A.sig + (size_t)req2 * A.sig_stride + P::n + (size_t)t2 * (P::a + 1) * P::n;
+                const uint32_t tg = p2.fors_idx[t2];
+                if (hgt < (uint32_t)P::a && x == ((tg >> hgt) ^ 1u))
+                    store_val<P::hash, NW>(rec2 + P::n + (size_t)hgt * P::n, out);
+                if (hgt == (uint32_t)P::a)
+                    store_val<P::hash, NW>(A.fors_roots + ((size_t)req2 * P::k + t2) * P::n, out);
+            }
+        }
+    }
+}
+
+// FORS public key: T_k over the k roots (fors.cu:142), lane per request
+template <class P>
+__global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr int NW = P::nw;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_req) return;
+    const Plan& pl = A.plan[i];
+    Key key;
+    Hh::load_key(pl, key);
+    Adr adr = adr_make(0, pl.tree, kForsPk);
+    adr_keypair<P::th>(adr, pl.leaf);
+    const uint8_t* roots = A.fors_roots + (size_t)i * P::k * P::n;
+    uint32_t out[NW];
+    typename Hh::Stream st;
+    if constexpr (P::hash == kSha2) {
+        st.begin(key, adr);
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(roots);
+#pragma unroll 1
+        for (int u = 0; u < P::k * NW; u++) st.word(bswap32(rw[u]));
+    } else {
+        st.begin(key, adr, reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(roots)), 1);
+    }
+    st.template finish<P::k * (P::hash == kSha2 ? NW : 1)>(out);
+    store_val<P::hash, NW>(A.fors_pk + (size_t)i * P::n, out);
+}
+
+// verify: one FORS tree per lane: leaf from the revealed sk, climb with the
+// auth path (fors_pk_from_sig, fors.cu:596-633; compute_root utils.cu:94-135)
+template <class P>
+__global__ void __launch_bounds__(64) k_vfors(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr int NW = P::nw;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= A.n_req * P::k) return;
+    const uint32_t req = g / P::k, t = g % P::k;
+    const Plan& pl = A.plan[req];
+    Key key;
+    Hh::load_key(pl, key);
+    const uint8_t* rec = A.sig + (size_t)req * A.sig_stride + P::n + (size_t)t * (P::a + 1) * P::n;
+    uint32_t li = pl.fors_idx[t];
+    uint32_t off = t << P::a;
+    Adr adr = adr_make(0, pl.tree, kForsTree);
+    adr_keypair<P::th>(adr, pl.leaf);
+    adr_node(adr, 0, li + off);
+    uint32_t sk[NW], node[NW];
+    load_val<P::hash, NW>(rec, sk);
+    Hh::F(key, adr, sk, node);
+#pragma unroll 1
+    for (int h = 0; h < P::a; h++) {
+        uint32_t ap[NW];
+        load_val<P::hash, NW>(rec + P::n + (size_t)h * P::n, ap);
+        const bool odd = li & 1u;
+        li >>= 1;
+        off >>= 1;
+        adr_node(adr, h + 1, li + off);
+        uint32_t l[NW], r[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            l[w] = odd ? ap[w] : node[w];
+            r[w] = odd ? node[w] : ap[w];
+        }
+        Hh::H(key, adr, l, r, node);
+    }
+    store_val<P::hash, NW>(A.fors_roots + (size_t)g * P::n, node);
+}
+
+// ------------------------------------------------------------- hypertree
+
+// WOTS chain lengths of an n-byte message (wots.cu:84-162): nibbles MSB
+// first, then the 3-nibble checksum (sum of 15 - d_i, shifted left by 4)
+template <class P>
+__device__ __forceinline__ uint32_t chain_len(const uint32_t msg[P::nw], int j) {
+    if (j < P::len1) {
+        const uint32_t byte = val_byte<P::hash>(msg, j >> 1);
+        return (j & 1) ? (byte & 15u) : (byte >> 4);
+    }
+    uint32_t csum = 0;
+#pragma unroll
+    for (int i = 0; i < P::n; i++) {
+        const uint32_t byte = val_byte<P::hash>(msg, i);
+        csum += 30u - (byte >> 4) - (byte & 15u);
+    }
+    const uint32_t v = csum << 4;  // 16-bit field, digits are its top 12 bits
+    const int c = j - P::len1;
+    return (v >> (12 - 4 * c)) & 15u;
+}
+
+// Hypertree leaves.  MODE 0 = sign: every (request, layer) subtree, auth
+// path into the signature, layer root into roots[], chain values of the
+// signing leaf captured.  MODE 1 = keygen: the top subtree only, root into
+// the key outputs.
+template <class P, int MODE>
+__global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr int NW = P::nw;
+    constexpr int LOGL = P::th;
+    constexpr int L = 1 << LOGL;
+    constexpr int WG = L > 256 ? L : 256;
+    constexpr int TPW = WG / L;
+    __shared__ uint32_t lds[2][WG][NW];
+
+    const int tid = threadIdx.x;
+    const uint32_t g = blockIdx.x * TPW + tid / L;
+    const uint32_t u = tid % L;  // leaf index = keypair address
+    const uint32_t ntrees = MODE == 0 ? A.n_req * P::d : A.n_req;
+    const bool active = g < ntrees;
+    const uint32_t req = active ? (MODE == 0 ? g / P::d : g) : 0;
+    const uint32_t layer = MODE == 0 ? (active ? g % P::d : 0) : P::d - 1;
+    const Plan& pl = A.plan[req];
+    Key key;
+    Hh::load_key(pl, key);
+    uint64_t tree = 0;
+    uint32_t target = 0;
+    if (MODE == 0) {
+        tree = layer == 0 ? pl.tree : (pl.tree >> (layer * P::th));
+        target = layer == 0 ? pl.leaf
+                            : (uint32_t)((pl.tree >> ((layer - 1) * P::th)) & (L - 1));
+    }
+    const bool capture = MODE == 0 && active && u == target;
+    uint8_t* cap = A.cap + ((size_t)req * P::d + layer) * P::len * 16 * P::n;
+
+    // ---- WOTS public key of leaf u, streamed into T_len (wots_gen_leaf, sign.cu:87-104)
+    Adr wa = adr_make(layer, tree, kWots);
+    adr_keypair<P::th>(wa, u);
+    Adr pa = adr_make(layer, tree, kWotsPk);
+    adr_keypair<P::th>(pa, u);
+    typename Hh::Stream ts;
+    if constexpr (P::hash == kSha2) {
+        ts.begin(key, pa);
+    } else {
+        const uint32_t lane = blockIdx.x * WG + tid;
+        ts.begin(key, pa, A.tl_buf + lane, A.tl_lanes);
+    }
+#pragma unroll 1
+    for (int j = 0; j < P::len; j++) {
+        uint32_t v[NW];
+        adr_chain(wa, j);
+        adr_hash(wa, 0);
+        Hh::PRF(key, wa, v);  // wots_gen_sk, wots.cu:27-33
+        if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16) * P::n, v);
+#pragma unroll 1
+        for (int s = 0; s < P::w - 1; s++) {  // gen_chain(0, w-1), wots.cu:51-62
+            adr_hash(wa, s);
+            Hh::F(key, wa, v, v);
+            if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16 + s + 1) * P::n, v);
+        }
+        if constexpr (P::hash == kSha2) {
+#pragma unroll 1
+            for (int i = 0; i < NW; i++) ts.word(pick<NW>(v, i));
+        } else {
+            ts.item(j, v);
+        }
+    }
+    uint32_t node[NW];
+    ts.template finish<P::hash == kSha2 ? P::len * NW : P::len>(node);
+
+    uint8_t* lsig = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
+                    (size_t)layer * P::layer_bytes;
+    uint8_t* auth = lsig + P::wots_bytes;
+    if (MODE == 0 && active && u == (target ^ 1u)) store_val<P::hash, NW>(auth, node);
+
+    // ---- subtree: level l has WG >> l nodes, node x = H(2x, 2x+1)
+#pragma unroll
+    for (int w = 0; w < NW; w++) lds[0][tid][w] = node[w];
+#pragma unroll
+    for (int l = 1; l <= LOGL; l++) {
+        __syncthreads();
+        const int src = (l - 1) & 1, dst = l & 1;
+        if (tid < (WG >> l)) {
+            const uint32_t tau = tid >> (LOGL - l);
+            const uint32_t x = tid & ((L >> l) - 1);
+            const uint32_t g2 = blockIdx.x * TPW + tau;
+            const bool act2 = g2 < ntrees;
+            const uint32_t req2 = act2 ? (MODE == 0 ? g2 / P::d : g2) : 0;
+            const uint32_t lay2 = MODE == 0 ? (act2 ? g2 % P::d : 0) : P::d - 1;
+            const Plan& p2 = A.plan[req2];
+            Key k2;
+            Hh::load_key(p2, k2);
+            uint64_t tr2 = 0;
+            uint32_t tg2 = 0;
+            if (MODE == 0) {
+                tr2 = lay2 == 0 ? p2.tree : (p2.tree >> (lay2 * P::th));
+                tg2 = lay2 == 0 ? p2.leaf
+                                : (uint32_t)((p2.tree >> ((lay2 - 1) * P::th)) & (L - 1));
+            }
+            uint32_t lft[NW], rgt[NW], out[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                lft[w] = lds[src][2 * tid][w];
+                rgt[w] = lds[src][2 * tid + 1][w];
+            }
+            Adr ha = adr_make(lay2, tr2, kHashTree);
+            adr_node(ha, l, x);
+            Hh::H(k2, ha, lft, rgt, out);
+#pragma unroll
+            for (int w = 0; w < NW; w++) lds[dst][tid][w] = out[w];
+            if (act2) {
+                if (MODE == 0) {
+                    uint8_t* a2 = A.sig + (size_t)req2 * A.sig_stride + P::n + P::fors_bytes +
+                                  (size_t)lay2 * P::layer_bytes + P::wots_bytes;
+                    if (l < LOGL && x == ((tg2 >> l) ^ 1u))
+                        store_val<P::hash, NW>(a2 + (size_t)l * P::n, out);
+                    if (l == LOGL)
+                        store_val<P::hash, NW>(A.roots + ((size_t)req2 * P::d + lay2) * P::n, out);
+                } else if (l == LOGL) {
+                    store_val<P::hash, NW>(A.sk_out + (size_t)req2 * P::sk_bytes + 3 * P::n, out);
+                    store_val<P::hash, NW>(A.pk_out + (size_t)req2 * P::pk_bytes + P::n, out);
+                }
+            }
+        }
+    }
+}
+
+// WOTS signatures: lane per (request, layer, chain); the layer signs the FORS
+// public key (layer 0) or the root of the layer below (sign.cu:1970-1989)
+template <class P>
+__global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
+    constexpr int NW = P::nw;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= A.n_req * P::d * P::len) return;
+    const uint32_t j = g % P::len;
+    const uint32_t rl = g / P::len;
+    const uint32_t layer = rl % P::d, req = rl / P::d;
+    const uint8_t* mp = layer == 0 ? A.fors_pk + (size_t)req * P::n
+                                   : A.roots + ((size_t)req * P::d + layer - 1) * P::n;
+    uint32_t msg[NW];
+    load_val<P::hash, NW>(mp, msg);
+    const uint32_t len = chain_len<P>(msg, j);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(
+        A.cap + ((((size_t)req * P::d + layer) * P::len + j) * 16 + len) * P::n);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(A.sig + (size_t)req * A.sig_stride + P::n +
+                                                P::fors_bytes + (size_t)layer * P::layer_bytes +
+                                                (size_t)j * P::n);
+#pragma unroll
+    for (int w = 0; w < NW; w++) dst[w] = src[w];
+}
+
+// verify: len lanes per request walk the d layers in order
+// (sign.cu:3077-3113): chains from the signature, T_len, compute_root
+template <class P>
+__global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr int NW = P::nw;
+    constexpr int LEN = P::len;
+    constexpr int G = 256 / LEN;  // requests per workgroup
+    __shared__ uint32_t pkbuf[G][LEN * NW];
+    __shared__ uint32_t rootb[G][NW];
+
+    const int tid = threadIdx.x;
+    const int grp = tid / LEN;
+    const int j = tid % LEN;
+    const uint32_t req = blockIdx.x * G + grp;
+    const bool active = grp < G && req < A.n_req;
+    const uint32_t rq = active ? req : 0;
+    const Plan& pl = A.plan[rq];
+    Key key;
+    Hh::load_key(pl, key);
+    const uint8_t* sig = A.sig + (size_t)rq * A.sig_stride;
+    if (active && j == 0) {
+        uint32_t fp[NW];
+        load_val<P::hash, NW>(A.fors_pk + (size_t)rq * P::n, fp);
+#pragma unroll
+        for (int w = 0; w < NW; w++) rootb[grp][w] = fp[w];
+    }
+    uint64_t tree = pl.tree;
+    uint32_t leaf = pl.leaf;
+#pragma unroll 1
+    for (int layer = 0; layer < P::d; layer++) {
+        __syncthreads();
+        const uint8_t* lsig = sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes;
+        if (active) {
+            uint32_t msg[NW], v[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) msg[w] = rootb[grp][w];
+            const uint32_t len = chain_len<P>(msg, j);
+            load_val<P::hash, NW>(lsig + (size_t)j * P::n, v);
+            Adr wa = adr_make(layer, tree, kWots);
+            adr_keypair<P::th>(wa, leaf);
+            adr_chain(wa, j);
+#pragma unroll 1
+            for (uint32_t s = len; s < (uint32_t)P::w - 1; s++) {
+                adr_hash(wa, s);
+                Hh::F(key, wa, v, v);
+            }
+#pragma unroll
+            for (int w = 0; w < NW; w++) pkbuf[grp][j * NW + w] = v[w];
+        }
+        __syncthreads();
+        if (active && j == 0) {
+            Adr pa = adr_make(layer, tree, kWotsPk);
+            adr_keypair<P::th>(pa, leaf);
+            uint32_t node[NW];
+            typename Hh::Stream ts;
+            if constexpr (P::hash == kSha2) {
+                ts.begin(key, pa);
+#pragma unroll 1
+                for (int q = 0; q < LEN * NW; q++) ts.word(pkbuf[grp][q]);
+                ts.template finish<LEN * NW>(node);
+            } else {
+                ts.begin(key, pa, &pkbuf[grp][0], 1);
+                ts.template finish<LEN>(node);
+            }
+            // compute_root over the subtree (utils.cu:94-135)
+            const uint8_t* auth = lsig + P::wots_bytes;
+            uint32_t li = leaf;
+            Adr ha = adr_make(layer, tree, kHashTree);
+#pragma unroll 1
+            for (int h = 0; h < P::th; h++) {
+                uint32_t ap[NW], l[NW], r[NW];
+                load_val<P::hash, NW>(auth + (size_t)h * P::n, ap);
+                const bool odd = li & 1u;
+                li >>= 1;
+                adr_node(ha, h + 1, li);
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    l[w] = odd ? ap[w] : node[w];
+                    r[w] = odd ? node[w] : ap[w];
+                }
+                Hh::H(key, ha, l, r, node);
+            }
+#pragma unroll
+            for (int w = 0; w < NW; w++) rootb[grp][w] = node[w];
+        }
+        leaf = (uint32_t)(tree & ((1u << P::th) - 1));
+        tree >>= P::th;
+    }
+    __syncthreads();
+    if (active && j == 0) {
+        bool ok = true;
+#pragma unroll
+        for (int w = 0; w < NW; w++) ok = ok && (rootb[grp][w] == pl.root[w]);
+        A.status[rq] = ok ? 0 : -1;
+    }
+}
+
+}  // namespace spx

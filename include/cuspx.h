@@ -1,0 +1,140 @@
+/*
+ * cuspx.h -- C ABI of the MI355X SPHINCS+ engine (libcuspx.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or
+ * a negative status:
+ *   -1  verification failed (api.h semantics)     -2  unknown parameter set
+ *   -3  HIP runtime / launch error                 -4  bad argument
+ *   -5  request exceeds the device-memory budget
+ *
+ * Parameter sets are chosen at run time (the reference fixes one per build
+ * with params.h macros).  Ids: 0..5 = sha2-{128s,128f,192s,192f,256s,256f},
+ * 6..11 = shake256-{128s,...,256f}.  The api.h names below act on the
+ * process-wide default set (sha2-128f unless CUSPX_PARAM names another set
+ * or cuspx_set_default_param() is called).
+ *
+ * DEBUG_MODE: the reference is shipped with DEBUG_MODE defined
+ * (all_option.h:74): keygen seeds become 0..3n-1 and optrand 0xad..ad after
+ * the DRBG bytes were drawn.  The library starts in that mode;
+ * cuspx_set_debug_mode(0) selects DRBG-derived seeds and optrand.
+ */
+#ifndef CUSPX_H
+#define CUSPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- set-up */
+
+typedef struct {
+    int pid, hash, n, full_h, d, a, k, len;
+    int sig_bytes, pk_bytes, sk_bytes, seed_bytes;
+    char name[32];
+} cuspx_param_info;
+
+int cuspx_num_params(void);
+int cuspx_param_id(const char *name);            /* "sha2-128f", ... ; -2 if unknown */
+int cuspx_get_param_info(int pid, cuspx_param_info *out);
+int cuspx_set_default_param(int pid);
+int cuspx_get_default_param(void);
+void cuspx_set_debug_mode(int on);
+int cuspx_get_debug_mode(void);
+const char *cuspx_last_error(void);
+
+/* ---------------------------------------------- api.h surface (rng.h, api.h) */
+
+/* replaces rng.h:48 randombytes_init / rng.cu:139 */
+void randombytes_init(unsigned char *entropy_input, unsigned char *personalization_string,
+                      int security_strength);
+/* replaces rng.h:53 randombytes / rng.cu:156 */
+int randombytes(unsigned char *x, unsigned long long xlen);
+
+/* replaces api.h:19-34 size getters / sign.cu:124-147 */
+unsigned long long crypto_sign_secretkeybytes(void);
+unsigned long long crypto_sign_publickeybytes(void);
+unsigned long long crypto_sign_bytes(void);
+unsigned long long crypto_sign_seedbytes(void);
+/* replaces api.h:41 / sign.cu:154 */
+int crypto_sign_seed_keypair(unsigned char *pk, unsigned char *sk, const unsigned char *seed);
+/* replaces api.h:48 / sign.cu:342 (draws 3n DRBG bytes) */
+int crypto_sign_keypair(unsigned char *pk, unsigned char *sk);
+/* replaces api.h:74 / sign.cu:1924 (draws n DRBG bytes of optrand) */
+int crypto_sign_signature(uint8_t *sig, size_t *siglen, const uint8_t *m, size_t mlen,
+                          const uint8_t *sk);
+/* replaces api.h:80 / sign.cu:3048 */
+int crypto_sign_verify(const uint8_t *sig, size_t siglen, const uint8_t *m, size_t mlen,
+                       const uint8_t *pk);
+/* replaces api.h:95 / sign.cu:3302 */
+int crypto_sign(unsigned char *sm, unsigned long long *smlen, const unsigned char *m,
+                unsigned long long mlen, const unsigned char *sk);
+/* replaces api.h:136 / sign.cu:6058 */
+int crypto_sign_open(unsigned char *m, unsigned long long *mlen, const unsigned char *sm,
+                     unsigned long long smlen, const unsigned char *pk);
+
+/* ------------------------------------- batched data-parallel surface (api.h) */
+
+/* replaces api.h:59 face_mdp_crypto_sign_keypair / sign.cu:1208.  num keys;
+ * pk[num][2n], sk[num][4n].  Seeds: 0..3n-1 in debug mode (as the reference),
+ * else 3n DRBG bytes per key in order (the reference draws curand). */
+int face_mdp_crypto_sign_keypair(unsigned char *pk, unsigned char *sk, unsigned int num);
+/* replaces api.h:107 face_mdp_crypto_sign / sign.cu:5098.  Message i is
+ * m + 32*i (SPX_MLEN stride) of mlen bytes; sm record i at stride
+ * SPX_BYTES+32 holds sig || m_i; *smlen = SPX_BYTES+32. */
+int face_mdp_crypto_sign(unsigned char *sm, unsigned long long *smlen, const unsigned char *m,
+                         unsigned long long mlen, const unsigned char *sk, unsigned int num);
+/* replaces api.h:142 face_mdp_crypto_sign_open / sign.cu:6856.  Records at
+ * stride SPX_BYTES+32, message length smlen-SPX_BYTES, output slots of 32
+ * bytes.  Returns 0 only if every item verifies (the reference reports
+ * item 0 only, sign.cu:6154); failed items get a zeroed slot. */
+int face_mdp_crypto_sign_open(unsigned char *m, unsigned long long *mlen, const unsigned char *sm,
+                              unsigned long long smlen, const unsigned char *pk, unsigned int num);
+
+/* ------------------------- runtime-parameterised batches with per-item status */
+
+/* seeds[num][3n] -> pk[num][2n], sk[num][4n] */
+int spx_batch_keygen(int pid, uint32_t num, const uint8_t *seeds, uint8_t *pk, uint8_t *sk);
+/* message i = msg + moff[i], mlen[i] bytes (any alignment, any length);
+ * optrand[num][n] or NULL (then drawn like crypto_sign_signature, item by
+ * item); sig[num][SPX_BYTES] */
+int spx_batch_sign(int pid, uint32_t num, const uint8_t *sk, const uint8_t *msg,
+                   const uint64_t *moff, const uint32_t *mlen, const uint8_t *optrand,
+                   uint8_t *sig);
+/* status[i] = 0 valid / -1 invalid; returns 0 if all valid, -1 otherwise */
+int spx_batch_verify(int pid, uint32_t num, const uint8_t *pk, const uint8_t *sig,
+                     const uint8_t *msg, const uint64_t *moff, const uint32_t *mlen,
+                     int32_t *status);
+
+/* ---------------------------- device-resident batches (inputs already in HBM)
+ * All pointers are device memory on the current HIP device; stream is a
+ * hipStream_t (NULL = the library's stream).  msg + moff[i] must be 4-byte
+ * aligned and readable up to the next 4-byte boundary.  ws is a workspace
+ * of cuspx_workspace_bytes(pid, num, op) bytes (op 0 sign, 1 verify,
+ * 2 keygen).  Calls are asynchronous on the stream. */
+size_t cuspx_workspace_bytes(int pid, uint32_t num, int op);
+int cuspx_sign_device(int pid, uint32_t num, const uint8_t *sk, uint32_t sk_stride,
+                      const uint8_t *msg, const uint64_t *moff, const uint32_t *mlen,
+                      const uint8_t *optrand, uint8_t *sig, uint64_t sig_stride, void *ws,
+                      void *stream);
+int cuspx_verify_device(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_stride,
+                        const uint8_t *sig, uint64_t sig_stride, const uint8_t *msg,
+                        const uint64_t *moff, const uint32_t *mlen, int32_t *status, void *ws,
+                        void *stream);
+int cuspx_keygen_device(int pid, uint32_t num, const uint8_t *seed, uint32_t seed_stride,
+                        uint8_t *pk, uint8_t *sk, void *ws, void *stream);
+
+/* ------------------------------------------------------ kernel timing hooks */
+/* When enabled, every launcher records a HIP event after each kernel on its
+ * stream.  cuspx_profile_read synchronises, then reports per kernel name the
+ * summed milliseconds and launch counts since the last reset. */
+void cuspx_profile_enable(int on);
+int cuspx_profile_read(char *names, int names_cap, float *ms, int *count, int cap);
+void cuspx_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""GPU parity: the HIP engine against the CPU oracle, byte for byte.
+
+Every case runs the HIP kernels through the C ABI (cuspx_amd -> libcuspx.so)
+on the same seeded inputs as the oracle: distinct keys per item, messages
+of uneven lengths, fixed optrand.  Invalid signatures must be rejected item
+by item (test/spx.cu:159-166 of the reference flips a message bit).
+"""
+import random
+
+import numpy as np
+import pytest
+
+import cuspx_amd as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F_SETS = ["sha2-128f", "sha2-192f", "sha2-256f", "shake256-128f", "shake256-192f",
+          "shake256-256f"]
+S_SETS = ["sha2-128s", "sha2-192s", "sha2-256s", "shake256-128s", "shake256-192s",
+          "shake256-256s"]
+
+
+def _inputs(pid, num, seed, lens):
+    p = O.params(pid)
+    rng = random.Random(seed)
+    seeds = [rng.randbytes(3 * p.n) for _ in range(num)]
+    msgs = [rng.randbytes(lens[i % len(lens)]) for i in range(num)]
+    opts = [rng.randbytes(p.n) for _ in range(num)]
+    return seeds, msgs, opts
+
+
+def _check_set(name, num, lens):
+    pid = C.param_id(name)
+    p = O.params(pid)
+    seeds, msgs, opts = _inputs(pid, num, 1000 + pid, lens)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    for i in range(num):
+        opk, osk = O.seed_keypair(pid, seeds[i])
+        assert bytes(pk[i]) == opk, (name, "pk", i)
+        assert bytes(sk[i]) == osk, (name, "sk", i)
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    for i in range(num):
+        want = O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i])
+        got = bytes(sig[i])
+        if got != want:
+            diff = [j for j in range(p.sig_bytes) if got[j] != want[j]]
+            pytest.fail("%s item %d: %d bytes differ, first at %d" % (name, i, len(diff), diff[0]))
+    st = C.batch_verify(pid, pk, sig, msgs)
+    assert list(st) == [0] * num
+    bad_msgs = list(msgs)
+    bad_msgs[0] = bytes([bad_msgs[0][0] ^ 1]) + bad_msgs[0][1:] if bad_msgs[0] else b"\x01"
+    bad_sig = sig.copy()
+    if num > 1:
+        bad_sig[1, p.n + 3] ^= 0x40  # a FORS secret-key byte
+    st = C.batch_verify(pid, pk, bad_sig, bad_msgs)
+    assert st[0] == -1
+    if num > 1:
+        assert st[1] == -1
+    assert list(st[2:]) == [0] * (num - 2)
+
+
+@pytest.mark.parametrize("name", F_SETS)
+def test_fast_sets_keygen_sign_verify(name):
+    _check_set(name, 6, [0, 1, 32, 33, 200, 4096])
+
+
+@pytest.mark.parametrize("name", S_SETS)
+def test_small_sets_keygen_sign_verify(name):
+    _check_set(name, 2, [32, 5])
+
+
+def test_sha2_128f_trace_roots_and_lengths_edge():
+    # message lengths around the SHA-256 block boundaries of PRF_msg / H_msg
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    lens = [0, 3, 4, 31, 47, 48, 55, 56, 63, 64, 65, 127, 128, 1000, 65536]
+    seeds, msgs, opts = _inputs(pid, len(lens), 77, lens)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    for i in range(len(lens)):
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), lens[i]
+    assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * len(lens)
+
+
+def test_api_surface_matches_oracle_debug_mode():
+    # as shipped (DEBUG_MODE): keygen seed 0..3n-1, optrand 0xad (sign.cu:345-349, 1950-1954)
+    pid = C.param_id("sha2-128f")
+    C.set_default_param(pid)
+    C.set_debug_mode(True)
+    O.set_debug_mode(True)
+    C.randombytes_init(bytes(range(48)))
+    O.randombytes_init(bytes(range(48)))
+    pk, sk = C.crypto_sign_keypair()
+    opk, osk = O.keypair(pid)
+    assert (pk, sk) == (opk, osk)
+    m = C.randombytes(33)
+    assert m == O.randombytes(33)
+    sm = C.crypto_sign(m, sk)
+    assert sm == O.sign(pid, m, osk)
+    rc, mo = C.crypto_sign_open(sm, pk)
+    assert rc == 0 and mo == m
+    bad = bytearray(sm)
+    bad[-1] ^= 1
+    rc, mo = C.crypto_sign_open(bytes(bad), pk)
+    assert rc == -1 and mo == b""
+    rc, mo = C.crypto_sign_open(sm[:100], pk)  # smlen < SPX_BYTES
+    assert rc == -1
+
+
+def test_api_surface_drbg_mode_kat_count0():
+    # KAT count 0 protocol (PQCgenKAT_sign.cu:55-131) in DRBG mode
+    pid = C.param_id("sha2-128f")
+    C.set_default_param(pid)
+    C.set_debug_mode(False)
+    O.set_debug_mode(False)
+    try:
+        C.randombytes_init(bytes(range(48)))
+        seed = C.randombytes(48)
+        msg = C.randombytes(33)
+        C.randombytes_init(seed)
+        O.randombytes_init(seed)
+        pk, sk = C.crypto_sign_keypair()
+        opk, osk = O.keypair(pid)
+        assert (pk, sk) == (opk, osk)
+        sm = C.crypto_sign(msg, sk)
+        assert sm == O.sign(pid, msg, osk)
+        assert C.crypto_sign_open(sm, pk) == (0, msg)
+    finally:
+        C.set_debug_mode(True)
+        O.set_debug_mode(True)
+
+
+def test_face_mdp_batch_surface():
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    C.set_default_param(pid)
+    C.set_debug_mode(True)
+    num = 5
+    pk, sk = C.face_mdp_crypto_sign_keypair(num)
+    opk, osk = O.seed_keypair(pid, bytes(range(3 * p.n)))
+    assert all(bytes(pk[i]) == opk and bytes(sk[i]) == osk for i in range(num))
+    rng = random.Random(5)
+    m = np.frombuffer(rng.randbytes(32 * num), np.uint8)
+    sm, smlen = C.face_mdp_crypto_sign(m, 32, sk, num)
+    assert smlen == p.sig_bytes + 32
+    for i in range(num):
+        mi = bytes(m[32 * i:32 * i + 32])
+        assert bytes(sm[i]) == O.sign(pid, mi, osk)
+    rc, mo, mlen = C.face_mdp_crypto_sign_open(sm, smlen, pk, num)
+    assert rc == 0 and mlen == 32
+    assert bytes(mo.reshape(-1)) == bytes(m)
+    sm2 = sm.copy()
+    sm2[2, -1] ^= 1
+    rc, mo, _ = C.face_mdp_crypto_sign_open(sm2, smlen, pk, num)
+    assert rc == -1
+    assert bytes(mo[2]) == bytes(32)
+    assert bytes(mo[1]) == bytes(m[32:64])
+
+
+def test_batch_determinism_and_order_independence():
+    pid = C.param_id("sha2-128f")
+    seeds, msgs, opts = _inputs(pid, 8, 9, [7, 32, 100])
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    opt = np.frombuffer(b"".join(opts), np.uint8).reshape(8, -1)
+    s1 = C.batch_sign(pid, sk, msgs, opt)
+    s2 = C.batch_sign(pid, sk, msgs, opt)
+    assert (s1 == s2).all()
+    perm = [3, 1, 7, 0, 5, 2, 6, 4]
+    s3 = C.batch_sign(pid, sk[perm], [msgs[i] for i in perm], opt[perm])
+    assert (s3 == s1[perm]).all()
