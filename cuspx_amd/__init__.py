@@ -89,6 +89,20 @@ def lib():
                                           ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
         L.cuspx_keygen_device.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
                                           vp, vp, vp]
+        for f in ("cuspx_dev_free", "cuspx_stream_destroy", "cuspx_stream_sync",
+                  "cuspx_event_destroy"):
+            getattr(L, f).argtypes = [vp]
+        L.cuspx_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        L.cuspx_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
+        L.cuspx_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
+        L.cuspx_memset_device.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
+        L.cuspx_stream_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.cuspx_event_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.cuspx_event_record.argtypes = [vp, vp]
+        L.cuspx_event_elapsed_ms.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float)]
+        L.cuspx_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p,
+                                        ctypes.c_int]
         L.cuspx_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -324,3 +338,87 @@ def profile_read():
     k = lib().cuspx_profile_read(names, 4096, ms, cnt, cap)
     keys = names.value.decode().split("\n")[:k]
     return {keys[i]: (ms[i], cnt[i]) for i in range(k)}
+
+
+# ------------------------------------------------------ device plumbing
+# The library carries its own HIP runtime; these helpers keep data resident
+# in HBM without a second runtime in the process.
+
+class DeviceBuffer:
+    """A device allocation owned by libcuspx's HIP runtime."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().cuspx_dev_alloc(ctypes.byref(p), self.nbytes), "cuspx_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(max(arr.nbytes, 1))
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().cuspx_memcpy_htod(self.ptr, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes),
+               "cuspx_memcpy_htod")
+
+    def download(self, dtype=np.uint8, count=None):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        _check(lib().cuspx_memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes),
+               "cuspx_memcpy_dtoh")
+        return out
+
+    def fill(self, value=0):
+        _check(lib().cuspx_memset_device(self.ptr, value, self.nbytes), "cuspx_memset_device")
+
+    def free(self):
+        if self.ptr:
+            lib().cuspx_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().cuspx_stream_create(ctypes.byref(p)), "cuspx_stream_create")
+        self.handle = p.value
+
+    def sync(self):
+        _check(lib().cuspx_stream_sync(self.handle), "cuspx_stream_sync")
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().cuspx_event_create(ctypes.byref(p)), "cuspx_event_create")
+        self.handle = p.value
+
+    def record(self, stream=None):
+        _check(lib().cuspx_event_record(self.handle, stream.handle if stream else None),
+               "cuspx_event_record")
+
+    def elapsed_ms(self, stop):
+        ms = ctypes.c_float()
+        _check(lib().cuspx_event_elapsed_ms(self.handle, stop.handle, ctypes.byref(ms)),
+               "cuspx_event_elapsed_ms")
+        return ms.value
+
+
+def device_info():
+    cu, clk, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    name = ctypes.create_string_buffer(128)
+    _check(lib().cuspx_device_info(ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(mem), name,
+                                   128), "cuspx_device_info")
+    return {"compute_units": cu.value, "clock_mhz": clk.value / 1000.0,
+            "hbm_gib": mem.value / 2**30, "name": name.value.decode()}

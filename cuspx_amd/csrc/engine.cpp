@@ -643,6 +643,75 @@ int cuspx_keygen_device(int pid, uint32_t num, const uint8_t* seed, uint32_t see
     return o->keygen(b, pick_stream(stream));
 }
 
+int cuspx_dev_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return kErrArg;
+    HIP_OK(hipMalloc(ptr, bytes ? bytes : 1));
+    return kOk;
+}
+int cuspx_dev_free(void* ptr) {
+    HIP_OK(hipFree(ptr));
+    return kOk;
+}
+int cuspx_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return kOk;
+}
+int cuspx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return kOk;
+}
+int cuspx_memset_device(void* dst, int value, size_t bytes) {
+    HIP_OK(hipMemset(dst, value, bytes));
+    return kOk;
+}
+int cuspx_stream_create(void** stream) {
+    if (!stream) return kErrArg;
+    hipStream_t s;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return kOk;
+}
+int cuspx_stream_destroy(void* stream) {
+    HIP_OK(hipStreamDestroy((hipStream_t)stream));
+    return kOk;
+}
+int cuspx_stream_sync(void* stream) {
+    HIP_OK(hipStreamSynchronize(pick_stream(stream)));
+    return kOk;
+}
+int cuspx_event_create(void** event) {
+    if (!event) return kErrArg;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    *event = e;
+    return kOk;
+}
+int cuspx_event_destroy(void* event) {
+    HIP_OK(hipEventDestroy((hipEvent_t)event));
+    return kOk;
+}
+int cuspx_event_record(void* event, void* stream) {
+    HIP_OK(hipEventRecord((hipEvent_t)event, pick_stream(stream)));
+    return kOk;
+}
+int cuspx_event_elapsed_ms(void* start, void* stop, float* ms) {
+    HIP_OK(hipEventSynchronize((hipEvent_t)stop));
+    HIP_OK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return kOk;
+}
+int cuspx_device_info(int* compute_units, int* clock_khz, size_t* hbm_bytes, char* name,
+                      int name_cap) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    HIP_OK(hipGetDeviceProperties(&pr, dev));
+    if (compute_units) *compute_units = pr.multiProcessorCount;
+    if (clock_khz) *clock_khz = pr.clockRate;
+    if (hbm_bytes) *hbm_bytes = pr.totalGlobalMem;
+    if (name && name_cap > 0) snprintf(name, name_cap, "%s (%s)", pr.name, pr.gcnArchName);
+    return kOk;
+}
+
 void cuspx_profile_enable(int on) {
     Prof& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);

@@ -382,6 +382,15 @@ __device__ __forceinline__ uint32_t chain_len(const uint32_t msg[P::nw], int j) 
     return (v >> (12 - 4 * c)) & 15u;
 }
 
+// tree index of hypertree layer `layer`: the reference shifts by h' once per
+// layer (sign.cu:1987-1988), so a total shift of 64 bits (256f, layer 16)
+// yields 0 -- a single 64-bit shift by 64 would be undefined
+template <class P>
+__device__ __forceinline__ uint64_t layer_tree(uint64_t tree, uint32_t layer) {
+    const uint32_t sh = layer * P::th;
+    return sh >= 64 ? 0ull : (tree >> sh);
+}
+
 // Hypertree leaves.  MODE 0 = sign: every (request, layer) subtree, auth
 // path into the signature, layer root into roots[], chain values of the
 // signing leaf captured.  MODE 1 = keygen: the top subtree only, root into
@@ -410,7 +419,7 @@ __global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(
     uint64_t tree = 0;
     uint32_t target = 0;
     if (MODE == 0) {
-        tree = layer == 0 ? pl.tree : (pl.tree >> (layer * P::th));
+        tree = layer_tree<P>(pl.tree, layer);
         target = layer == 0 ? pl.leaf
                             : (uint32_t)((pl.tree >> ((layer - 1) * P::th)) & (L - 1));
     }
@@ -477,7 +486,7 @@ __global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(
             uint64_t tr2 = 0;
             uint32_t tg2 = 0;
             if (MODE == 0) {
-                tr2 = lay2 == 0 ? p2.tree : (p2.tree >> (lay2 * P::th));
+                tr2 = layer_tree<P>(p2.tree, lay2);
                 tg2 = lay2 == 0 ? p2.leaf
                                 : (uint32_t)((p2.tree >> ((lay2 - 1) * P::th)) & (L - 1));
             }

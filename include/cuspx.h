@@ -126,6 +126,26 @@ int cuspx_verify_device(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_st
 int cuspx_keygen_device(int pid, uint32_t num, const uint8_t *seed, uint32_t seed_stride,
                         uint8_t *pk, uint8_t *sk, void *ws, void *stream);
 
+/* ------------------------------------------------------ device plumbing
+ * Minimal device-memory, stream and event helpers over the library's own
+ * HIP runtime, so callers that hold no HIP runtime of their own (Python via
+ * ctypes) can keep batches resident in HBM and time them. */
+int cuspx_dev_alloc(void **ptr, size_t bytes);
+int cuspx_dev_free(void *ptr);
+int cuspx_memcpy_htod(void *dst, const void *src, size_t bytes);
+int cuspx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int cuspx_memset_device(void *dst, int value, size_t bytes);
+int cuspx_stream_create(void **stream);
+int cuspx_stream_destroy(void *stream);
+int cuspx_stream_sync(void *stream);
+int cuspx_event_create(void **event);
+int cuspx_event_destroy(void *event);
+int cuspx_event_record(void *event, void *stream);
+int cuspx_event_elapsed_ms(void *start, void *stop, float *ms);
+/* compute units, peak engine clock (kHz), HBM bytes, device name */
+int cuspx_device_info(int *compute_units, int *clock_khz, size_t *hbm_bytes, char *name,
+                      int name_cap);
+
 /* ------------------------------------------------------ kernel timing hooks */
 /* When enabled, every launcher records a HIP event after each kernel on its
  * stream.  cuspx_profile_read synchronises, then reports per kernel name the
