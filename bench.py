@@ -21,7 +21,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12   # MI355X: 256 CUs x 64 lanes x 2.4 GHz
-SHA256_OPS = 64 * 14 + 48 * 10 + 8        # VALU ops of one generic compression on gfx950
+# VALU lane-ops per compression of the shipped k_ht: SQ_INSTS_VALU x 64 over the
+# compressions of one launch (profiles/k_ht_sha2_128f_b4096.md); the generic
+# FIPS 180-4 compression with v_alignbit / v_bitop3 / v_add3 is 64*14+48*10+8 = 1384
+SHA256_OPS = 1232
+SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
 KECCAK_OPS = 24 * 182                      # VALU ops of one Keccak-f[1600] on 32-bit halves
 
 
@@ -182,6 +186,15 @@ def main():
     sign_s = sum(sign_ms) / len(sign_ms) / 1e3
     ver_s = sum(ver_ms) / len(ver_ms) / 1e3
 
+    traffic = None
+    try:
+        tj = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))
+        t = tj.get("k_ht/%s/%d" % (p.name.decode(), B))
+        if t:
+            traffic = int(1024 * (t["fetch_kib"] * t["fetch_scale"] +
+                                  t["write_kib"] * t["write_scale"]))
+    except (OSError, ValueError):
+        pass
     kt, kn = prof.get("k_ht", (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
     work = kernel_work(p, B)
@@ -210,11 +223,15 @@ def main():
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
         "roofline": {"bound": "valu", "kernel": "k_ht", "achieved": round(achieved, 3),
                      "peak": round(PEAK_INT_TOPS, 2), "unit": "TOP/s",
-                     "frac": round(achieved / PEAK_INT_TOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_INT_TOPS, 4), "traffic": traffic,
                      "work_per_launch": work,
                      "work_unit": "sha256 compressions" if p.hash == 0 else "keccak-f permutations",
                      "ops_per_unit": SHA256_OPS if p.hash == 0 else KECCAK_OPS,
-                     "kernel_ms": round(k_avg * 1e3, 3)},
+                     "kernel_ms": round(k_avg * 1e3, 3),
+                     "units_per_s": round(work / k_avg, 1) if k_avg > 0 else 0.0,
+                     "frac_at_1450_ops": round(work / k_avg / (PEAK_INT_TOPS * 1e12 /
+                                                                SHA256_OPS_SURVEY), 4)
+                     if (k_avg > 0 and p.hash == 0) else None},
         "cpu_baseline": cpu,
         "device": info,
     }

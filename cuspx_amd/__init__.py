@@ -95,6 +95,7 @@ def lib():
         L.cuspx_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         L.cuspx_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
         L.cuspx_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
+        L.cuspx_memcpy_dtod.argtypes = [vp, vp, ctypes.c_size_t]
         L.cuspx_memset_device.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
         L.cuspx_stream_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
         L.cuspx_event_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]

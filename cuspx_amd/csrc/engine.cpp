@@ -660,6 +660,10 @@ int cuspx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return kOk;
 }
+int cuspx_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return kOk;
+}
 int cuspx_memset_device(void* dst, int value, size_t bytes) {
     HIP_OK(hipMemset(dst, value, bytes));
     return kOk;

@@ -134,6 +134,7 @@ int cuspx_dev_alloc(void **ptr, size_t bytes);
 int cuspx_dev_free(void *ptr);
 int cuspx_memcpy_htod(void *dst, const void *src, size_t bytes);
 int cuspx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int cuspx_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int cuspx_memset_device(void *dst, int value, size_t bytes);
 int cuspx_stream_create(void **stream);
 int cuspx_stream_destroy(void *stream);
