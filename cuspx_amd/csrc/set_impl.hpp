@@ -107,7 +107,8 @@ struct SetImpl {
         A.n_req = b.n;
         Lay::carve(A, (uint8_t*)b.ws, b.n, kOpVerify);
         const uint32_t n = b.n;
-        constexpr uint32_t groups = 256 / P::len;
+        constexpr int kVerifyLanes = 16;  // lanes per request in k_vht
+        constexpr uint32_t groups = 256 / kVerifyLanes;
         prof_begin(s);
         hipLaunchKernelGGL((k_vprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_vprep");
@@ -115,7 +116,7 @@ struct SetImpl {
         prof_mark(s, "k_vfors");
         hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
-        hipLaunchKernelGGL((k_vht<P>), dim3(cdiv(n, groups)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((k_vht<P, kVerifyLanes>), dim3(cdiv(n, groups)), dim3(256), 0, s, A);
         prof_mark(s, "k_vht");
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;
     }

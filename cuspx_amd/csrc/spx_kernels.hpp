@@ -542,29 +542,36 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     for (int w = 0; w < NW; w++) dst[w] = src[w];
 }
 
-// verify: len lanes per request walk the d layers in order
-// (sign.cu:3077-3113): chains from the signature, T_len, compute_root
-template <class P>
+// verify: G lanes per request walk the d layers in order (sign.cu:3077-3113).
+// Per layer the len WOTS chains (15 - len_j steps each, data dependent) are
+// ranked by remaining steps in LDS and dealt to the G lanes in snake order,
+// a longest-first schedule that evens the per-lane F counts; each lane runs
+// its chains in one flattened loop (one F site).  One lane per request then
+// computes T_len and climbs the subtree (compute_root, utils.cu:94-135).
+template <class P, int G>
 __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     constexpr int NW = P::nw;
     constexpr int LEN = P::len;
-    constexpr int G = 256 / LEN;  // requests per workgroup
-    __shared__ uint32_t pkbuf[G][LEN * NW];
-    __shared__ uint32_t rootb[G][NW];
+    constexpr int R = 256 / G;                 // requests per workgroup
+    constexpr int MAXC = (LEN + G - 1) / G;    // chains per lane
+    __shared__ uint32_t pkbuf[R][LEN * NW];
+    __shared__ uint32_t rootb[R][NW];
+    __shared__ uint8_t steps[R][LEN];
+    __shared__ uint8_t order[R][LEN];
 
     const int tid = threadIdx.x;
-    const int grp = tid / LEN;
-    const int j = tid % LEN;
-    const uint32_t req = blockIdx.x * G + grp;
-    const bool active = grp < G && req < A.n_req;
+    const int grp = tid / G;
+    const int l = tid % G;
+    const uint32_t req = blockIdx.x * R + grp;
+    const bool active = req < A.n_req;
     const uint32_t rq = active ? req : 0;
     const Plan& pl = A.plan[rq];
     Key key;
     Hh::load_key(pl, key);
     const uint8_t* sig = A.sig + (size_t)rq * A.sig_stride;
-    if (active && j == 0) {
+    if (l == 0) {
         uint32_t fp[NW];
         load_val<P::hash, NW>(A.fors_pk + (size_t)rq * P::n, fp);
 #pragma unroll
@@ -576,25 +583,76 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     for (int layer = 0; layer < P::d; layer++) {
         __syncthreads();
         const uint8_t* lsig = sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes;
-        if (active) {
-            uint32_t msg[NW], v[NW];
+        uint32_t msg[NW];
 #pragma unroll
-            for (int w = 0; w < NW; w++) msg[w] = rootb[grp][w];
-            const uint32_t len = chain_len<P>(msg, j);
-            load_val<P::hash, NW>(lsig + (size_t)j * P::n, v);
-            Adr wa = adr_make(layer, tree, kWots);
-            adr_keypair<P::th>(wa, leaf);
-            adr_chain(wa, j);
-#pragma unroll 1
-            for (uint32_t s = len; s < (uint32_t)P::w - 1; s++) {
-                adr_hash(wa, s);
-                Hh::F(key, wa, v, v);
-            }
+        for (int w = 0; w < NW; w++) msg[w] = rootb[grp][w];
+        // remaining steps per chain
 #pragma unroll
-            for (int w = 0; w < NW; w++) pkbuf[grp][j * NW + w] = v[w];
+        for (int c = 0; c < MAXC; c++) {
+            const int j = l + c * G;
+            if (j < LEN) steps[grp][j] = (uint8_t)(P::w - 1 - chain_len<P>(msg, j));
         }
         __syncthreads();
-        if (active && j == 0) {
+        // rank: more steps first, ties by chain index
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const int j = l + c * G;
+            if (j < LEN) {
+                const uint32_t sj = steps[grp][j];
+                int rank = 0;
+#pragma unroll 5
+                for (int i = 0; i < LEN; i++) {
+                    const uint32_t si = steps[grp][i];
+                    rank += (si > sj) || (si == sj && i < j);
+                }
+                order[grp][rank] = (uint8_t)j;
+            }
+        }
+        __syncthreads();
+        // my chains: ranks l, 2G-1-l, 2G+l, 4G-1-l, ... (snake order)
+        int mine[MAXC];
+        uint32_t pre[MAXC][NW];
+        int nm = 0;
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const int r = (c & 1) ? (c + 1) * G - 1 - l : c * G + l;
+            mine[c] = r < LEN ? order[grp][r] : 0;
+            nm += r < LEN;
+            load_val<P::hash, NW>(lsig + (size_t)mine[c] * P::n, pre[c]);
+        }
+        Adr wa = adr_make(layer, tree, kWots);
+        adr_keypair<P::th>(wa, leaf);
+        int c = 0;
+        int j = mine[0];
+        uint32_t s = P::w - 1 - steps[grp][j];
+        uint32_t v[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) v[w] = pre[0][w];
+#pragma unroll 1
+        while (active && c < nm) {
+            if (s >= (uint32_t)P::w - 1) {  // chain done: park it, take the next
+#pragma unroll
+                for (int w = 0; w < NW; w++) pkbuf[grp][j * NW + w] = v[w];
+                c++;
+                if (c < nm) {
+#pragma unroll
+                    for (int cc = 1; cc < MAXC; cc++)
+                        if (c == cc) {
+                            j = mine[cc];
+#pragma unroll
+                            for (int w = 0; w < NW; w++) v[w] = pre[cc][w];
+                        }
+                    s = P::w - 1 - steps[grp][j];
+                }
+                continue;
+            }
+            adr_chain(wa, j);
+            adr_hash(wa, s);
+            Hh::F(key, wa, v, v);
+            s++;
+        }
+        __syncthreads();
+        if (active && l == 0) {
             Adr pa = adr_make(layer, tree, kWotsPk);
             adr_keypair<P::th>(pa, leaf);
             uint32_t node[NW];
@@ -608,23 +666,22 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
                 ts.begin(key, pa, &pkbuf[grp][0], 1);
                 ts.template finish<LEN>(node);
             }
-            // compute_root over the subtree (utils.cu:94-135)
             const uint8_t* auth = lsig + P::wots_bytes;
             uint32_t li = leaf;
             Adr ha = adr_make(layer, tree, kHashTree);
 #pragma unroll 1
             for (int h = 0; h < P::th; h++) {
-                uint32_t ap[NW], l[NW], r[NW];
+                uint32_t ap[NW], lft[NW], rgt[NW];
                 load_val<P::hash, NW>(auth + (size_t)h * P::n, ap);
                 const bool odd = li & 1u;
                 li >>= 1;
                 adr_node(ha, h + 1, li);
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
-                    l[w] = odd ? ap[w] : node[w];
-                    r[w] = odd ? node[w] : ap[w];
+                    lft[w] = odd ? ap[w] : node[w];
+                    rgt[w] = odd ? node[w] : ap[w];
                 }
-                Hh::H(key, ha, l, r, node);
+                Hh::H(key, ha, lft, rgt, node);
             }
 #pragma unroll
             for (int w = 0; w < NW; w++) rootb[grp][w] = node[w];
@@ -633,7 +690,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
         tree >>= P::th;
     }
     __syncthreads();
-    if (active && j == 0) {
+    if (active && l == 0) {
         bool ok = true;
 #pragma unroll
         for (int w = 0; w < NW; w++) ok = ok && (rootb[grp][w] == pl.root[w]);
