@@ -95,6 +95,111 @@ def make_inputs(C, pid, batch):
     return to(kseeds), to(opts), to(msgs)
 
 
+SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
+
+
+def serve_phase(C, seconds, qps, sat_requests, single_requests, seed=5):
+    """Serving workload: 256 resident keys over SHA2-128f / SHA2-128s /
+    SHAKE256-256f, Poisson arrivals at `qps`, 80% sign / 20% verify, message
+    sizes log-uniform in 16 B .. 64 KiB.  Returns latency percentiles at the
+    fixed QPS, the saturated throughput of the batched server and the
+    throughput of plain per-request launches on the same mix."""
+    import random
+    import numpy as np
+    rng = random.Random(seed)
+    C.randombytes_init(bytes([seed]) * 48)
+    keys = []
+    for s_i, name in enumerate(SERVE_SETS):
+        pid = C.param_id(name)
+        p = C.params(pid)
+        cnt = 86 if s_i < 2 else 84
+        pk, sk = C.batch_keygen(pid, np.frombuffer(C.randombytes(3 * p.n * cnt), np.uint8))
+        keys += [(pid, bytes(sk[i]), bytes(pk[i])) for i in range(cnt)]
+    pool = np.random.default_rng(seed).integers(0, 256, 1 << 20, dtype=np.uint8).tobytes()
+
+    def msg():
+        n = int(round(np.exp(rng.uniform(np.log(16), np.log(65536)))))
+        o = rng.randrange(0, len(pool) - n)
+        return pool[o:o + n]
+
+    # signatures for the verify requests: 2 messages per key
+    signed = []
+    for pid in sorted(set(k[0] for k in keys)):
+        idx = [i for i, k in enumerate(keys) if k[0] == pid]
+        ms = [msg() for _ in idx for _ in range(2)]
+        sks = np.frombuffer(b"".join(keys[i][1] for i in idx for _ in range(2)), np.uint8)
+        sigs = C.batch_sign(pid, sks, ms)
+        signed += [(idx[j // 2], ms[j], bytes(sigs[j])) for j in range(len(ms))]
+
+    def mix(n):
+        out = []
+        for _ in range(n):
+            if rng.random() < 0.8:
+                out.append((0, rng.randrange(len(keys)), msg(), None))
+            else:
+                k, m, s = signed[rng.randrange(len(signed))]
+                out.append((1, k, m, s))
+        return out
+
+    def run(srv, kids, reqs, gaps):
+        tickets = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for (op, k, m, s), g in zip(reqs, gaps):
+            t_next += g
+            while True:
+                d = t_next - time.perf_counter()
+                if d <= 0:
+                    break
+                if d > 2e-4:
+                    time.sleep(d - 1.5e-4)
+            t = srv.submit_sign(kids[k], m) if op == 0 else srv.submit_verify(kids[k], m, s)
+            tickets.append((t, op))
+        bad = 0
+        for t, op in tickets:
+            st, _ = srv.wait(t)
+            bad += st != 0
+        el = time.perf_counter() - t0
+        return el, bad
+
+    out = {"sets": SERVE_SETS, "keys": len(keys), "mix": "80% sign / 20% verify",
+           "message_bytes": "log-uniform 16..65536"}
+    with C.Server(hbm_budget_bytes=24 << 30, slots=4, batch_window_us=2000) as srv:
+        kids = [srv.add_key(pid, sk, pk) for pid, sk, pk in keys]
+        n = int(qps * seconds)
+        reqs = mix(n)
+        gaps = [rng.expovariate(qps) for _ in range(n)]
+        el, bad = run(srv, kids, reqs, gaps)
+        st = srv.stats()
+        out.update({"qps_offered": qps, "requests": n, "seconds": round(el, 3),
+                    "throughput_rps": round(n / el, 1), "invalid": bad,
+                    "sign_p50_ms": round(st["sign_p50_us"] / 1e3, 3),
+                    "sign_p99_ms": round(st["sign_p99_us"] / 1e3, 3),
+                    "verify_p50_ms": round(st["verify_p50_us"] / 1e3, 3),
+                    "verify_p99_ms": round(st["verify_p99_us"] / 1e3, 3),
+                    "mean_batch": round(st["mean_batch"], 2)})
+        srv.reset_stats()
+        reqs = mix(sat_requests)
+        el, bad2 = run(srv, kids, reqs, [0.0] * len(reqs))
+        st = srv.stats()
+        out.update({"saturation_rps": round(sat_requests / el, 1),
+                    "saturation_sign_p99_ms": round(st["sign_p99_us"] / 1e3, 3),
+                    "saturation_mean_batch": round(st["mean_batch"], 2)})
+        bad += bad2
+    with C.Server(hbm_budget_bytes=4 << 30, slots=1, max_batch=1) as one:
+        kids = [one.add_key(pid, sk, pk) for pid, sk, pk in keys]
+        reqs = mix(single_requests)
+        t0 = time.perf_counter()
+        for op, k, m, s in reqs:
+            t = one.submit_sign(kids[k], m) if op == 0 else one.submit_verify(kids[k], m, s)
+            bad += one.wait(t)[0] != 0
+        el = time.perf_counter() - t0
+        out["per_request_launch_rps"] = round(single_requests / el, 1)
+    out["invalid"] = bad
+    out["speedup_vs_per_request"] = round(out["saturation_rps"] / out["per_request_launch_rps"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +210,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-qps", type=float, default=5000.0)
+    ap.add_argument("--serve-saturation", type=int, default=8000)
+    ap.add_argument("--serve-single", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -195,6 +304,12 @@ def main():
                                   t["write_kib"] * t["write_scale"]))
     except (OSError, ValueError):
         pass
+    serve = None
+    if args.serve_seconds > 0:
+        print("serving: %.0f s at %.0f req/s" % (args.serve_seconds, args.serve_qps),
+              file=sys.stderr, flush=True)
+        serve = serve_phase(C, args.serve_seconds, args.serve_qps, args.serve_saturation,
+                            args.serve_single)
     kt, kn = prof.get("k_ht", (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
     work = kernel_work(p, B)
@@ -218,6 +333,9 @@ def main():
                                % (p.name.decode(), B, B - 1),
                    "param": p.name.decode(), "batch": B},
         "verifies_per_s": round(B / ver_s, 1),
+        "sign_p50_ms": serve["sign_p50_ms"] if serve else None,
+        "sign_p99_ms": serve["sign_p99_ms"] if serve else None,
+        "serve_qps": serve["qps_offered"] if serve else None,
         "sign_batch_ms": round(sign_s * 1e3, 3),
         "verify_batch_ms": round(ver_s * 1e3, 3),
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
@@ -233,6 +351,7 @@ def main():
                                                                 SHA256_OPS_SURVEY), 4)
                      if (k_avg > 0 and p.hash == 0) else None},
         "cpu_baseline": cpu,
+        "serve": serve,
         "device": info,
     }
     print(json.dumps(out), flush=True)

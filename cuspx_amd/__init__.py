@@ -423,3 +423,102 @@ def device_info():
                                    128), "cuspx_device_info")
     return {"compute_units": cu.value, "clock_mhz": clk.value / 1000.0,
             "hbm_gib": mem.value / 2**30, "name": name.value.decode()}
+
+
+# ---------------------------------------------------------------- serving
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
+                ("slots", ctypes.c_uint32), ("max_msg_bytes", ctypes.c_uint32),
+                ("batch_window_us", ctypes.c_uint32), ("max_keys_per_set", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("signs", ctypes.c_uint64), ("verifies", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("pending", ctypes.c_uint64),
+                ("mean_batch", ctypes.c_double), ("sign_p50_us", ctypes.c_float),
+                ("sign_p99_us", ctypes.c_float), ("verify_p50_us", ctypes.c_float),
+                ("verify_p99_us", ctypes.c_float), ("hbm_bytes", ctypes.c_uint64),
+                ("last_error", ctypes.c_int32)]
+
+
+class Server:
+    """Continuous-batching sign/verify service with keys resident on the GPU
+    (libcuspx spx_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=0, max_batch=0, slots=0, max_msg_bytes=0,
+                 batch_window_us=200, max_keys_per_set=0):
+        L = lib()
+        vp = ctypes.c_void_p
+        L.spx_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        L.spx_server_add_key.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(ctypes.c_int32)]
+        L.spx_server_submit.argtypes = [vp, ctypes.c_int, ctypes.c_int32, vp, ctypes.c_uint32, vp,
+                                        ctypes.POINTER(ctypes.c_uint64)]
+        L.spx_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int, vp,
+                                      ctypes.POINTER(ctypes.c_int32)]
+        L.spx_server_cancel.argtypes = [vp, ctypes.c_uint64]
+        L.spx_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.spx_server_reset_stats.argtypes = [vp]
+        L.spx_server_destroy.argtypes = [vp]
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, max_msg_bytes, batch_window_us,
+                           max_keys_per_set)
+        h = ctypes.c_void_p()
+        _check(L.spx_server_create(ctypes.byref(cfg), ctypes.byref(h)), "spx_server_create")
+        self.h = h.value
+        self._sig_bytes = {}
+
+    def add_key(self, pid, sk, pk):
+        kid = ctypes.c_int32()
+        skb = None if sk is None else ctypes.create_string_buffer(bytes(sk))
+        _check(lib().spx_server_add_key(self.h, pid, skb, ctypes.create_string_buffer(bytes(pk)),
+                                        ctypes.byref(kid)), "spx_server_add_key")
+        self._sig_bytes[kid.value] = params(pid).sig_bytes
+        return kid.value
+
+    def _submit(self, op, key, msg, aux):
+        t = ctypes.c_uint64()
+        m = np.frombuffer(msg, np.uint8) if len(msg) else np.zeros(1, np.uint8)
+        a = None if aux is None else np.frombuffer(bytes(aux), np.uint8)
+        rc = lib().spx_server_submit(self.h, op, key, _ptr(m), len(msg), _ptr(a), ctypes.byref(t))
+        if rc != OK:
+            raise CuspxError("spx_server_submit failed (%d)" % rc)
+        return t.value
+
+    def submit_sign(self, key, msg, optrand=None):
+        return self._submit(0, key, msg, optrand)
+
+    def submit_verify(self, key, msg, sig):
+        return self._submit(1, key, msg, sig)
+
+    def wait(self, ticket, timeout_ms=-1, key=None):
+        """(status, signature-or-None); status 0 ok / -1 invalid signature."""
+        st = ctypes.c_int32()
+        out = ctypes.create_string_buffer(64 * 1024)
+        rc = lib().spx_server_wait(self.h, ticket, timeout_ms, out, ctypes.byref(st))
+        if rc == 1:
+            raise TimeoutError("request %d still running" % ticket)
+        _check(rc, "spx_server_wait")
+        n = self._sig_bytes.get(key, 0) if key is not None else 0
+        return st.value, (out.raw[:n] if n else out.raw)
+
+    def cancel(self, ticket):
+        return lib().spx_server_cancel(self.h, ticket)
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().spx_server_stats(self.h, ctypes.byref(s)), "spx_server_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        lib().spx_server_reset_stats(self.h)
+
+    def close(self):
+        if self.h:
+            lib().spx_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -155,7 +155,7 @@ void prof_begin(hipStream_t s) {
     std::lock_guard<std::mutex> lk(p.mu);
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, s);
+    (void)hipEventRecord(e, s);
     p.calls.push_back({{"", e}});
     p.open[s] = p.calls.size() - 1;
 }
@@ -168,7 +168,7 @@ void prof_mark(hipStream_t s, const char* kernel) {
     if (it == p.open.end()) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, s);
+    (void)hipEventRecord(e, s);
     p.calls[it->second].push_back({kernel, e});
 }
 
@@ -726,7 +726,7 @@ void cuspx_profile_reset(void) {
     Prof& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     for (auto& c : p.calls)
-        for (auto& m : c) hipEventDestroy(m.ev);
+        for (auto& m : c) (void)hipEventDestroy(m.ev);
     p.calls.clear();
     p.open.clear();
     p.acc.clear();
@@ -736,15 +736,15 @@ int cuspx_profile_read(char* names, int names_cap, float* ms, int* count, int ca
     Prof& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     for (auto& c : p.calls) {
-        for (size_t i = 0; i < c.size(); i++) hipEventSynchronize(c[i].ev);
+        for (size_t i = 0; i < c.size(); i++) (void)hipEventSynchronize(c[i].ev);
         for (size_t i = 1; i < c.size(); i++) {
             float t = 0.f;
-            hipEventElapsedTime(&t, c[i - 1].ev, c[i].ev);
+            (void)hipEventElapsedTime(&t, c[i - 1].ev, c[i].ev);
             auto& a = p.acc[c[i].name];
             a.first += t;
             a.second += 1;
         }
-        for (auto& m : c) hipEventDestroy(m.ev);
+        for (auto& m : c) (void)hipEventDestroy(m.ev);
     }
     p.calls.clear();
     p.open.clear();

@@ -32,6 +32,8 @@ struct DevBatch {
     const uint64_t* moff;
     const uint32_t* mlen;
     const uint8_t* optrand;  // [n][param n]
+    const uint32_t* key_idx; // optional key row per request (sk / pk tables)
+    const uint32_t* key_mid; // optional precomputed midstate per key row (8 words)
     uint8_t* sig;
     uint64_t sig_stride;
     int32_t* status;
@@ -49,6 +51,9 @@ struct SetOps {
     int (*sign)(const DevBatch&, hipStream_t);
     int (*verify)(const DevBatch&, hipStream_t);
     int (*keygen)(const DevBatch&, hipStream_t);
+    // mid[i] = SHA-256 midstate of public key row i (zeros for SHAKE sets)
+    int (*midstates)(const uint8_t* pk, uint32_t pk_stride, uint32_t* mid, uint32_t n,
+                     hipStream_t);
 };
 
 const SetOps* get_set(int pid);

@@ -71,6 +71,8 @@ struct SetImpl {
         A.moff = b.moff;
         A.mlen = b.mlen;
         A.optrand = b.optrand;
+        A.key_idx = b.key_idx;
+        A.key_mid = b.key_mid;
         A.sig = b.sig;
         A.sig_stride = b.sig_stride;
         A.n_req = b.n;
@@ -98,6 +100,8 @@ struct SetImpl {
         BatchArgs A{};
         A.pk = b.pk;
         A.pk_stride = b.pk_stride;
+        A.key_idx = b.key_idx;
+        A.key_mid = b.key_mid;
         A.msg = b.msg;
         A.moff = b.moff;
         A.mlen = b.mlen;
@@ -138,6 +142,14 @@ struct SetImpl {
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;
     }
 
+    static int midstates(const uint8_t* pk, uint32_t pk_stride, uint32_t* mid, uint32_t n,
+                         hipStream_t s) {
+        if (n == 0) return 0;
+        hipLaunchKernelGGL((k_midstate<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, pk, pk_stride, mid,
+                           n);
+        return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+    }
+
     static SetOps ops(const char* name, int pid) {
         SetOps o{};
         o.name = name;
@@ -156,6 +168,7 @@ struct SetImpl {
         o.sign = &sign;
         o.verify = &verify;
         o.keygen = &keygen;
+        o.midstates = &midstates;
         return o;
     }
 };

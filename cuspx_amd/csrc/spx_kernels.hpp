@@ -44,6 +44,8 @@ struct BatchArgs {
     const uint64_t* moff;
     const uint32_t* mlen;
     const uint8_t* optrand;   // sign: [N][n]
+    const uint32_t* key_idx;  // optional: request i uses key row key_idx[i] of sk / pk
+    const uint32_t* key_mid;  // optional: precomputed SHA-256 midstate per key row (8 words)
     // outputs
     uint8_t* sig;             // signatures (sign: written, verify: read)
     uint64_t sig_stride;
@@ -98,7 +100,8 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
     constexpr int NW = P::nw;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n_req) return;
-    const uint8_t* sk = A.sk + (size_t)i * A.sk_stride;
+    const uint32_t row = A.key_idx ? A.key_idx[i] : i;
+    const uint8_t* sk = A.sk + (size_t)row * A.sk_stride;
     uint32_t skseed[NW], skprf[NW], pub[NW], pk[2 * NW], opt[NW], R[NW];
     load_val<P::hash, NW>(sk, skseed);
     load_val<P::hash, NW>(sk + P::n, skprf);
@@ -108,7 +111,12 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
     const uint8_t* m = A.msg + A.moff[i];
     const uint32_t ml = A.mlen[i];
     Plan pl;
-    Hh::midstate(pub, pl.mid);
+    if (A.key_mid) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
+    } else {
+        Hh::midstate(pub, pl.mid);
+    }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         pl.skseed[j] = j < NW ? skseed[j] : 0u;
@@ -132,14 +140,20 @@ __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
     constexpr int NW = P::nw;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n_req) return;
-    const uint8_t* pkp = A.pk + (size_t)i * A.pk_stride;
+    const uint32_t row = A.key_idx ? A.key_idx[i] : i;
+    const uint8_t* pkp = A.pk + (size_t)row * A.pk_stride;
     uint32_t pub[NW], pk[2 * NW], R[NW];
     load_val<P::hash, NW>(pkp, pub);
     load_val<P::hash, 2 * NW>(pkp, pk);
     const uint8_t* sig = A.sig + (size_t)i * A.sig_stride;
     load_val<P::hash, NW>(sig, R);
     Plan pl;
-    Hh::midstate(pub, pl.mid);
+    if (A.key_mid) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
+    } else {
+        Hh::midstate(pub, pl.mid);
+    }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         pl.skseed[j] = 0;
@@ -151,6 +165,22 @@ __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
     plan_from_digest<P>(pl, dg);
     pl.status = 0;
     A.plan[i] = pl;
+}
+
+// resident-key midstates: mid[i] = state after pub_seed || 0^(64-n) for the
+// public key row i (seed_state, sha256.cu:391-405); zero for SHAKE sets
+template <class P>
+__global__ void __launch_bounds__(64) k_midstate(const uint8_t* pk, uint32_t pk_stride,
+                                                 uint32_t* mid, uint32_t n) {
+    using Hh = HashT<P>;
+    constexpr int NW = P::nw;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pub[NW], m[8];
+    load_val<P::hash, NW>(pk + (size_t)i * pk_stride, pub);
+    Hh::midstate(pub, m);
+#pragma unroll
+    for (int j = 0; j < 8; j++) mid[(size_t)i * 8 + j] = m[j];
 }
 
 // keygen: sk = seed (3n) || root, plan with the key material
@@ -253,7 +283,7 @@ __global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
     // cross-lane reduction: level l holds WG >> l nodes, node x = H(2x, 2x+1)
 #pragma unroll
     for (int w = 0; w < NW; w++) lds[0][tid][w] = node[w];
-#pragma unroll
+#pragma unroll 1
     for (int l = 1; l <= LOGL; l++) {
         __syncthreads();
         const int src = (l - 1) & 1, dst = l & 1;
@@ -469,7 +499,7 @@ __global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(
     // ---- subtree: level l has WG >> l nodes, node x = H(2x, 2x+1)
 #pragma unroll
     for (int w = 0; w < NW; w++) lds[0][tid][w] = node[w];
-#pragma unroll
+#pragma unroll 1
     for (int l = 1; l <= LOGL; l++) {
         __syncthreads();
         const int src = (l - 1) & 1, dst = l & 1;

@@ -126,6 +126,48 @@ int cuspx_verify_device(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_st
 int cuspx_keygen_device(int pid, uint32_t num, const uint8_t *seed, uint32_t seed_stride,
                         uint8_t *pk, uint8_t *sk, void *ws, void *stream);
 
+/* ------------------------------------------------------------ serving
+ * Continuous-batching service: keys resident in HBM (sk / pk rows and the
+ * SHA-256 pub_seed midstate per key), sign / verify requests of any
+ * parameter set and message length queued per (set, op), up to `slots`
+ * batches in flight on separate streams, batches formed as slots free up and
+ * bounded by the device-memory budget.  Not in the reference (it has no
+ * serving path); sign results equal crypto_sign_signature for the same key,
+ * message and optrand. */
+typedef struct {
+    uint64_t hbm_budget_bytes;  /* device memory for batches (0: min(free/4, 16 GiB)) */
+    uint32_t max_batch;         /* requests per launch (0: 4096) */
+    uint32_t slots;             /* batches in flight (0: 4) */
+    uint32_t max_msg_bytes;     /* largest message accepted (0: 65536) */
+    uint32_t batch_window_us;   /* longest wait for a partial batch while the GPU is busy */
+    uint32_t max_keys_per_set;  /* resident keys per parameter set (0: 1024) */
+} spx_server_config;
+
+typedef struct {
+    uint64_t signs, verifies, batches, pending;
+    double mean_batch;
+    float sign_p50_us, sign_p99_us, verify_p50_us, verify_p99_us;
+    uint64_t hbm_bytes;
+    int32_t last_error;
+} spx_server_statistics;
+
+typedef struct spx_server spx_server;
+int spx_server_create(const spx_server_config *cfg, spx_server **out);
+/* sk may be NULL for verify-only keys; key_id = pid * 65536 + row */
+int spx_server_add_key(spx_server *s, int pid, const uint8_t *sk, const uint8_t *pk,
+                       int32_t *key_id);
+/* op 0 = sign (aux = optrand[n] or NULL to draw it like crypto_sign_signature),
+ * op 1 = verify (aux = signature).  -5 if the request can never fit the budget */
+int spx_server_submit(spx_server *s, int op, int32_t key_id, const uint8_t *msg, uint32_t mlen,
+                      const uint8_t *aux, uint64_t *ticket);
+/* 0 done (status: 0 ok / -1 invalid signature), 1 timeout, <0 error */
+int spx_server_wait(spx_server *s, uint64_t ticket, int timeout_ms, uint8_t *sig_out,
+                    int32_t *status);
+int spx_server_cancel(spx_server *s, uint64_t ticket);
+int spx_server_stats(spx_server *s, spx_server_statistics *st);
+int spx_server_reset_stats(spx_server *s);
+int spx_server_destroy(spx_server *s);
+
 /* ------------------------------------------------------ device plumbing
  * Minimal device-memory, stream and event helpers over the library's own
  * HIP runtime, so callers that hold no HIP runtime of their own (Python via

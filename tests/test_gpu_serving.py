@@ -1,0 +1,91 @@
+"""GPU tests of the continuous-batching server: keys resident on the GPU,
+mixed parameter sets and message sizes; every signature must be identical
+whether its request ran alone or inside a mixed stream of concurrent
+requests, and equal to the oracle's."""
+import random
+
+import numpy as np
+import pytest
+
+import cuspx_amd as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
+
+
+def _keys():
+    out = []
+    rng = random.Random(31)
+    for name in SETS:
+        pid = C.param_id(name)
+        p = O.params(pid)
+        seeds = [rng.randbytes(3 * p.n) for _ in range(2)]
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+        for i in range(2):
+            out.append((pid, bytes(sk[i]), bytes(pk[i])))
+    return out
+
+
+def test_server_alone_equals_mixed_and_oracle():
+    keys = _keys()
+    rng = random.Random(32)
+    with C.Server(hbm_budget_bytes=2 << 30, slots=3, batch_window_us=500) as srv:
+        kids = [srv.add_key(pid, sk, pk) for pid, sk, pk in keys]
+        reqs = []
+        for i in range(24):
+            k = i % len(keys)
+            pid = keys[k][0]
+            p = O.params(pid)
+            mlen = int(round(np.exp(rng.uniform(np.log(16), np.log(4096)))))
+            reqs.append((k, rng.randbytes(mlen), rng.randbytes(p.n)))
+        alone = []
+        for k, m, opt in reqs[:6]:
+            st, sig = srv.wait(srv.submit_sign(kids[k], m, opt), key=kids[k])
+            assert st == 0
+            alone.append(sig)
+        # oracle for one request of each parameter set (requests 0, 2, 4)
+        for (k, m, opt), sig in list(zip(reqs[:6], alone))[:6:2]:
+            pid, sk, pk = keys[k]
+            assert sig == O.signature_with_optrand(pid, m, sk, opt)
+        # the same requests inside a mixed stream, plus verifies in flight
+        tickets = [(srv.submit_sign(kids[k], m, opt), k) for k, m, opt in reqs]
+        vt = [(srv.submit_verify(kids[k], reqs[j][1], alone[j]), j)
+              for j, (k, _, _) in enumerate(reqs[:6])]
+        bad = bytearray(alone[0])
+        bad[40] ^= 1
+        vbad = srv.submit_verify(kids[reqs[0][0]], reqs[0][1], bytes(bad))
+        mixed = [srv.wait(t, key=kids[k]) for t, k in tickets]
+        assert all(st == 0 for st, _ in mixed)
+        for j in range(6):
+            assert mixed[j][1] == alone[j], j
+        for t, j in vt:
+            assert srv.wait(t)[0] == 0
+        assert srv.wait(vbad)[0] == -1
+        # every mixed signature verifies with the batch API
+        for (k, m, opt), (st, sig) in zip(reqs, mixed):
+            pid, sk, pk = keys[k]
+            assert list(C.batch_verify(pid, np.frombuffer(pk, np.uint8),
+                                       np.frombuffer(sig, np.uint8), [m])) == [0]
+        s = srv.stats()
+        assert s["signs"] >= 30 and s["verifies"] == 7
+        assert s["sign_p99_us"] > 0
+
+
+def test_server_admission_and_cancel():
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    pk, sk = C.batch_keygen(pid, np.zeros(3 * p.n, np.uint8))
+    with C.Server(hbm_budget_bytes=1 << 30, slots=2, max_msg_bytes=1024) as srv:
+        kid = srv.add_key(pid, bytes(sk[0]), bytes(pk[0]))
+        with pytest.raises(C.CuspxError):
+            srv.submit_sign(kid, b"x" * 2048)  # above max_msg_bytes: refused
+        with pytest.raises(C.CuspxError):
+            srv.submit_sign(kid + 7, b"x")  # unknown key row
+        ts = [srv.submit_sign(kid, b"m%d" % i) for i in range(20)]
+        srv.cancel(ts[5])
+        for i, t in enumerate(ts):
+            if i == 5:
+                continue
+            assert srv.wait(t, key=kid)[0] == 0
