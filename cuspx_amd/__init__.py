@@ -142,6 +142,17 @@ def set_default_param(pid):
         raise KeyError(pid)
 
 
+def set_split(pid, op, lanes_per_leaf):
+    """Force the hypertree lanes-per-leaf split (0 = automatic / tuned)."""
+    rc = lib().cuspx_set_split(pid, op, lanes_per_leaf)
+    if rc != OK:
+        raise CuspxError("cuspx_set_split failed (%d)" % rc)
+
+
+def get_split(pid, op, num):
+    return lib().cuspx_get_split(pid, op, num)
+
+
 def set_debug_mode(on):
     lib().cuspx_set_debug_mode(1 if on else 0)
 
@@ -511,6 +522,18 @@ class Server:
 
     def reset_stats(self):
         lib().spx_server_reset_stats(self.h)
+
+    def latency(self, pid, op):
+        """(p50_us, p99_us, count) of one (parameter set, op)."""
+        L = lib()
+        L.spx_server_latency.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_uint64)]
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_uint64()
+        _check(L.spx_server_latency(self.h, pid, op, ctypes.byref(a), ctypes.byref(b),
+                                    ctypes.byref(c)), "spx_server_latency")
+        return a.value, b.value, c.value
 
     def close(self):
         if self.h:

@@ -9,8 +9,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
+
+#include <fstream>
 #include <map>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -47,6 +51,68 @@ int num_sets() { return 12; }
 const SetOps* get_set(int pid) {
     if (pid < 0 || pid >= 12) return nullptr;
     return kSetFns[pid]();
+}
+
+// ------------------------------------------------------------ tuning table
+//
+// Text file of `<set> <sign|keygen> <min_subtrees> <lanes_per_leaf>` lines
+// written by the autotuner (python -m cuspx_amd.autotune); read once from
+// $CUSPX_TUNING or tuning.txt next to libcuspx.so.  For a launch over T
+// subtrees the entry with the largest min_subtrees <= T applies.
+
+struct Tuning {
+    std::mutex mu;
+    bool loaded = false;
+    std::map<std::pair<int, int>, std::map<uint32_t, int>> split;
+    std::map<std::pair<int, int>, int> forced;  // cuspx_set_split overrides
+
+    void load() {
+        std::string path;
+        if (const char* env = getenv("CUSPX_TUNING")) {
+            path = env;
+        } else {
+            Dl_info info;
+            if (dladdr((void*)&tuned_split, &info) && info.dli_fname) {
+                path = info.dli_fname;
+                path = path.substr(0, path.find_last_of('/') + 1) + "tuning.txt";
+            }
+        }
+        std::ifstream f(path);
+        std::string line;
+        while (std::getline(f, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream ls(line);
+            std::string name, op;
+            uint32_t min_trees = 0;
+            int S = 0;
+            if (!(ls >> name >> op >> min_trees >> S)) continue;
+            int pid = -1;
+            for (int i = 0; i < 12; i++)
+                if (name == get_set(i)->name) pid = i;
+            const int o = op == "sign" ? kOpSign : (op == "keygen" ? kOpKeygen : -1);
+            if (pid < 0 || o < 0) continue;
+            split[{pid, o}][min_trees] = S;
+        }
+        loaded = true;
+    }
+};
+
+static Tuning& tuning() {
+    static Tuning t;
+    return t;
+}
+
+int tuned_split(int pid, int op, uint32_t trees) {
+    Tuning& t = tuning();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (!t.loaded) t.load();
+    auto f = t.forced.find({pid, op});
+    if (f != t.forced.end()) return f->second;
+    auto it = t.split.find({pid, op});
+    if (it == t.split.end()) return 0;
+    auto e = it->second.upper_bound(trees);
+    if (e == it->second.begin()) return 0;
+    return std::prev(e)->second;
 }
 
 // ------------------------------------------------------------ error text
@@ -400,6 +466,25 @@ int cuspx_get_param_info(int pid, cuspx_param_info* out) {
     out->seed_bytes = 3 * o->n;
     snprintf(out->name, sizeof(out->name), "%s", o->name);
     return kOk;
+}
+
+int cuspx_set_split(int pid, int op, int lanes_per_leaf) {
+    if (!get_set(pid)) return kErrParam;
+    if ((op != kOpSign && op != kOpKeygen) ||
+        (lanes_per_leaf != 0 && lanes_per_leaf != 1 && lanes_per_leaf != 2 &&
+         lanes_per_leaf != 4 && lanes_per_leaf != 8))
+        return kErrArg;
+    Tuning& t = tuning();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (lanes_per_leaf == 0) t.forced.erase({pid, op});
+    else t.forced[{pid, op}] = lanes_per_leaf;
+    return kOk;
+}
+
+int cuspx_get_split(int pid, int op, uint32_t num) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    return o->split(op, num);
 }
 
 int cuspx_set_default_param(int pid) {

@@ -95,6 +95,7 @@ struct Server {
     uint64_t pending_bytes = 0;
     // statistics
     std::vector<float> lat_us[2];
+    std::map<std::pair<int, int>, std::vector<float>> lat_set;
     uint64_t batches = 0, batched_reqs = 0;
 
     // bytes one request of (pid, op) needs in a slot besides its message
@@ -196,7 +197,10 @@ void Server::harvest(Slot& sl) {
         r.t_done = now;
         if (!r.cancelled) {
             const float us = std::chrono::duration<float, std::micro>(now - r.t_submit).count();
-            if (lat_us[sl.op].size() < (1u << 24)) lat_us[sl.op].push_back(us);
+            if (lat_us[sl.op].size() < (1u << 24)) {
+                lat_us[sl.op].push_back(us);
+                lat_set[{sl.pid, sl.op}].push_back(us);
+            }
         } else {
             live.erase(r.ticket);
         }
@@ -479,7 +483,25 @@ int spx_server_reset_stats(spx_server* sv) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat_us[0].clear();
     s->lat_us[1].clear();
+    s->lat_set.clear();
     s->batches = s->batched_reqs = 0;
+    return kOk;
+}
+
+int spx_server_latency(spx_server* sv, int pid, int op, float* p50_us, float* p99_us,
+                       uint64_t* count) {
+    Server* s = (Server*)sv;
+    if (!s || !p50_us || !p99_us || !count) return kErrArg;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->lat_set.find({pid, op});
+    if (it == s->lat_set.end()) {
+        *p50_us = *p99_us = 0.f;
+        *count = 0;
+        return kOk;
+    }
+    *p50_us = pct(it->second, 0.50);
+    *p99_us = pct(it->second, 0.99);
+    *count = it->second.size();
     return kOk;
 }
 

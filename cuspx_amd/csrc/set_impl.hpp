@@ -9,15 +9,40 @@ namespace spx {
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// lanes that fill the chip: 1024 SIMDs x 64 lanes x 4 waves
+static constexpr uint64_t kFillLanes = 1024ull * 64 * 4;
+
 // workspace carve-up shared by sign / verify / keygen
 template <class P>
 struct Layout {
     static constexpr uint64_t al(uint64_t x) { return (x + 255) & ~uint64_t(255); }
-    static uint32_t ht_block(void) { return (1u << P::th) > 256 ? (1u << P::th) : 256u; }
-    static uint32_t ht_trees_per_block(void) { return ht_block() >> P::th; }
-    static uint32_t ht_blocks(uint32_t trees) { return cdiv(trees, ht_trees_per_block()); }
-    static uint64_t tl_lanes(uint32_t trees) { return (uint64_t)ht_blocks(trees) * ht_block(); }
-    static uint64_t bytes(uint32_t n, int op) {
+
+    // lanes per WOTS leaf for a k_ht launch over `trees` subtrees: a tuned
+    // value when the tuning table has one, else double S while the launch
+    // would still fill less than half the chip (S = 1 for large batches)
+    static int split(int pid, int op, uint32_t trees) {
+        const int t = tuned_split(pid, op, trees);
+        if (t == 1 || t == 2 || t == 4 || t == 8) return t;
+        const uint64_t leaves = (uint64_t)trees << P::th;
+        int S = 1;
+        while (S < 8 && leaves * S * 2 <= kFillLanes) S *= 2;
+        return S;
+    }
+    template <int S>
+    static uint32_t ht_blocks(uint32_t trees) { return cdiv(trees, HtShape<P, S>::TPW); }
+    template <int S>
+    static uint64_t tl_cols(uint32_t trees) {
+        return HtShape<P, S>::park ? (uint64_t)ht_blocks<S>(trees) * HtShape<P, S>::NL : 0;
+    }
+    static uint64_t tl_cols(uint32_t trees, int S) {
+        switch (S) {
+            case 1: return tl_cols<1>(trees);
+            case 2: return tl_cols<2>(trees);
+            case 4: return tl_cols<4>(trees);
+            default: return tl_cols<8>(trees);
+        }
+    }
+    static uint64_t bytes(int pid, uint32_t n, int op) {
         uint64_t b = al((uint64_t)n * sizeof(Plan));
         b += al((uint64_t)n * P::k * P::n);   // fors roots
         b += al((uint64_t)n * P::n);          // fors pk
@@ -25,13 +50,17 @@ struct Layout {
             b += al((uint64_t)n * P::d * P::n);                    // layer roots
             b += al((uint64_t)n * P::d * P::len * 16 * P::n);      // chain capture
         }
-        if (P::hash == kShake && op != kOpVerify) {
+        if (op != kOpVerify) {
+            // sized for every split, so a workspace stays valid whichever S runs
             const uint32_t trees = op == kOpSign ? n * P::d : n;
-            b += al(tl_lanes(trees) * P::len * P::nw * 4);
+            uint64_t cols = 0;
+            for (int S = 1; S <= 8; S *= 2) cols = tl_cols(trees, S) > cols ? tl_cols(trees, S) : cols;
+            b += al(cols * P::len * P::nw * 4);
         }
+        (void)pid;
         return b + 256;
     }
-    static void carve(BatchArgs& A, uint8_t* ws, uint32_t n, int op) {
+    static void carve(int pid, BatchArgs& A, uint8_t* ws, uint32_t n, int op, int S) {
         uint8_t* p = (uint8_t*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
         A.plan = (Plan*)p;
         p += al((uint64_t)n * sizeof(Plan));
@@ -49,11 +78,12 @@ struct Layout {
             A.cap = p;
             p += al((uint64_t)n * P::d * P::len * 16 * P::n);
         }
-        if (P::hash == kShake && op != kOpVerify) {
+        if (op != kOpVerify) {
             const uint32_t trees = op == kOpSign ? n * P::d : n;
             A.tl_buf = (uint32_t*)p;
-            A.tl_lanes = (uint32_t)tl_lanes(trees);
+            A.tl_lanes = (uint32_t)tl_cols(trees, S);
         }
+        (void)pid;
     }
 };
 
@@ -61,7 +91,23 @@ template <class P, int J>
 struct SetImpl {
     using Lay = Layout<P>;
 
-    static size_t workspace(uint32_t n, int op) { return (size_t)Lay::bytes(n, op); }
+    static int pid_;
+    static size_t workspace(uint32_t n, int op) { return (size_t)Lay::bytes(pid_, n, op); }
+
+    template <int MODE, int S>
+    static void launch_ht(const BatchArgs& A, uint32_t trees, hipStream_t s) {
+        hipLaunchKernelGGL((k_ht<P, MODE, S>), dim3(Lay::template ht_blocks<S>(trees)),
+                           dim3(HtShape<P, S>::WG), 0, s, A);
+    }
+    template <int MODE>
+    static void launch_ht(const BatchArgs& A, uint32_t trees, int S, hipStream_t s) {
+        switch (S) {
+            case 1: launch_ht<MODE, 1>(A, trees, s); break;
+            case 2: launch_ht<MODE, 2>(A, trees, s); break;
+            case 4: launch_ht<MODE, 4>(A, trees, s); break;
+            default: launch_ht<MODE, 8>(A, trees, s); break;
+        }
+    }
 
     static int sign(const DevBatch& b, hipStream_t s) {
         BatchArgs A{};
@@ -76,8 +122,9 @@ struct SetImpl {
         A.sig = b.sig;
         A.sig_stride = b.sig_stride;
         A.n_req = b.n;
-        Lay::carve(A, (uint8_t*)b.ws, b.n, kOpSign);
         const uint32_t n = b.n;
+        const int S = Lay::split(pid_, kOpSign, n * P::d);
+        Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpSign, S);
         constexpr uint32_t fors_tpw = 256u >> (P::a - J);
         prof_begin(s);
         hipLaunchKernelGGL((k_prep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
@@ -87,8 +134,7 @@ struct SetImpl {
         prof_mark(s, "k_fors");
         hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
-        hipLaunchKernelGGL((k_ht<P, 0>), dim3(Lay::ht_blocks(n * P::d)), dim3(Lay::ht_block()), 0,
-                           s, A);
+        launch_ht<0>(A, n * P::d, S, s);
         prof_mark(s, "k_ht");
         hipLaunchKernelGGL((k_wots_gather<P>), dim3(cdiv((uint64_t)n * P::d * P::len, 256)),
                            dim3(256), 0, s, A);
@@ -109,7 +155,7 @@ struct SetImpl {
         A.sig_stride = b.sig_stride;
         A.status = b.status;
         A.n_req = b.n;
-        Lay::carve(A, (uint8_t*)b.ws, b.n, kOpVerify);
+        Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpVerify, 1);
         const uint32_t n = b.n;
         constexpr int kVerifyLanes = 16;  // lanes per request in k_vht
         constexpr uint32_t groups = 256 / kVerifyLanes;
@@ -132,12 +178,13 @@ struct SetImpl {
         A.pk_out = b.pk_out;
         A.sk_out = b.sk_out;
         A.n_req = b.n;
-        Lay::carve(A, (uint8_t*)b.ws, b.n, kOpKeygen);
         const uint32_t n = b.n;
+        const int S = Lay::split(pid_, kOpKeygen, n);
+        Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpKeygen, S);
         prof_begin(s);
         hipLaunchKernelGGL((k_kprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_kprep");
-        hipLaunchKernelGGL((k_ht<P, 1>), dim3(Lay::ht_blocks(n)), dim3(Lay::ht_block()), 0, s, A);
+        launch_ht<1>(A, n, S, s);
         prof_mark(s, "k_ht_keygen");
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;
     }
@@ -150,7 +197,12 @@ struct SetImpl {
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;
     }
 
+    static int split(int op, uint32_t n) {
+        return Lay::split(pid_, op, op == kOpSign ? n * P::d : n);
+    }
+
     static SetOps ops(const char* name, int pid) {
+        pid_ = pid;
         SetOps o{};
         o.name = name;
         o.pid = pid;
@@ -169,9 +221,13 @@ struct SetImpl {
         o.verify = &verify;
         o.keygen = &keygen;
         o.midstates = &midstates;
+        o.split = &split;
         return o;
     }
 };
+
+template <class P, int J>
+int SetImpl<P, J>::pid_ = -1;
 
 }  // namespace spx
 

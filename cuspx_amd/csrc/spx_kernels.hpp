@@ -425,20 +425,38 @@ __device__ __forceinline__ uint64_t layer_tree(uint64_t tree, uint32_t layer) {
 // path into the signature, layer root into roots[], chain values of the
 // signing leaf captured.  MODE 1 = keygen: the top subtree only, root into
 // the key outputs.
-template <class P, int MODE>
-__global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(BatchArgs A) {
+//
+// S lanes share one WOTS leaf (lane q takes chains q, q+S, ...): S = 1 gives
+// the most work per lane (throughput), larger S shortens the per-lane chain
+// of dependent compressions (latency of small batches); the launcher picks S.
+// With S > 1 (and always for SHAKE) the chain outputs are parked in tl_buf
+// ([word][leaf], coalesced) and lane q = 0 absorbs them into T_len.
+template <class P, int S>
+struct HtShape {
+    static constexpr int L = 1 << P::th;          // leaves per subtree
+    static constexpr int LT = L * S;              // lanes per subtree
+    static constexpr int WG = LT > 256 ? LT : 256;
+    static constexpr int TPW = WG / LT;           // subtrees per workgroup
+    static constexpr int NL = WG / S;             // leaves per workgroup
+    static constexpr bool park = S > 1 || P::hash == kShake;
+};
+
+template <class P, int MODE, int S>
+__global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
+    using Sh = HtShape<P, S>;
     constexpr int NW = P::nw;
     constexpr int LOGL = P::th;
-    constexpr int L = 1 << LOGL;
-    constexpr int WG = L > 256 ? L : 256;
-    constexpr int TPW = WG / L;
-    __shared__ uint32_t lds[2][WG][NW];
+    constexpr int L = Sh::L, LT = Sh::LT, TPW = Sh::TPW, NL = Sh::NL;
+    __shared__ uint32_t lds[2][NL][NW];
 
     const int tid = threadIdx.x;
-    const uint32_t g = blockIdx.x * TPW + tid / L;
-    const uint32_t u = tid % L;  // leaf index = keypair address
+    const int tau = tid / LT;
+    const uint32_t u = (tid % LT) / S;  // leaf index = keypair address
+    const int q = tid % S;              // chain phase of this lane
+    const int slot = tau * L + u;       // leaf slot in the workgroup
+    const uint32_t g = blockIdx.x * TPW + tau;
     const uint32_t ntrees = MODE == 0 ? A.n_req * P::d : A.n_req;
     const bool active = g < ntrees;
     const uint32_t req = active ? (MODE == 0 ? g / P::d : g) : 0;
@@ -455,8 +473,9 @@ __global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(
     }
     const bool capture = MODE == 0 && active && u == target;
     uint8_t* cap = A.cap + ((size_t)req * P::d + layer) * P::len * 16 * P::n;
+    const uint32_t leaf_g = blockIdx.x * NL + slot;  // leaf column in tl_buf
 
-    // ---- WOTS public key of leaf u, streamed into T_len (wots_gen_leaf, sign.cu:87-104)
+    // ---- WOTS public key of leaf u (wots_gen_leaf, sign.cu:87-104)
     Adr wa = adr_make(layer, tree, kWots);
     adr_keypair<P::th>(wa, u);
     Adr pa = adr_make(layer, tree, kWotsPk);
@@ -465,11 +484,10 @@ __global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(
     if constexpr (P::hash == kSha2) {
         ts.begin(key, pa);
     } else {
-        const uint32_t lane = blockIdx.x * WG + tid;
-        ts.begin(key, pa, A.tl_buf + lane, A.tl_lanes);
+        ts.begin(key, pa, A.tl_buf + leaf_g, A.tl_lanes);
     }
 #pragma unroll 1
-    for (int j = 0; j < P::len; j++) {
+    for (int j = q; j < P::len; j += S) {
         uint32_t v[NW];
         adr_chain(wa, j);
         adr_hash(wa, 0);
@@ -481,32 +499,43 @@ __global__ void __launch_bounds__((1 << P::th) > 256 ? (1 << P::th) : 256) k_ht(
             Hh::F(key, wa, v, v);
             if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16 + s + 1) * P::n, v);
         }
-        if constexpr (P::hash == kSha2) {
+        if constexpr (!Sh::park) {
 #pragma unroll 1
             for (int i = 0; i < NW; i++) ts.word(pick<NW>(v, i));
         } else {
-            ts.item(j, v);
+#pragma unroll
+            for (int i = 0; i < NW; i++) A.tl_buf[(size_t)(j * NW + i) * A.tl_lanes + leaf_g] = v[i];
         }
     }
+    if constexpr (S > 1) __syncthreads();  // all chains of the leaf parked
     uint32_t node[NW];
-    ts.template finish<P::hash == kSha2 ? P::len * NW : P::len>(node);
-
-    uint8_t* lsig = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
-                    (size_t)layer * P::layer_bytes;
-    uint8_t* auth = lsig + P::wots_bytes;
-    if (MODE == 0 && active && u == (target ^ 1u)) store_val<P::hash, NW>(auth, node);
-
-    // ---- subtree: level l has WG >> l nodes, node x = H(2x, 2x+1)
+    if (q == 0) {
+        if constexpr (P::hash == kSha2) {
+            if constexpr (Sh::park) {
+#pragma unroll 1
+                for (int w = 0; w < P::len * NW; w++)
+                    ts.word(A.tl_buf[(size_t)w * A.tl_lanes + leaf_g]);
+            }
+            ts.template finish<P::len * NW>(node);
+        } else {
+            ts.template finish<P::len>(node);
+        }
+        uint8_t* auth = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
+                        (size_t)layer * P::layer_bytes + P::wots_bytes;
+        if (MODE == 0 && active && u == (target ^ 1u)) store_val<P::hash, NW>(auth, node);
 #pragma unroll
-    for (int w = 0; w < NW; w++) lds[0][tid][w] = node[w];
+        for (int w = 0; w < NW; w++) lds[0][slot][w] = node[w];
+    }
+
+    // ---- subtree: level l has NL >> l nodes, node x = H(2x, 2x+1)
 #pragma unroll 1
     for (int l = 1; l <= LOGL; l++) {
         __syncthreads();
         const int src = (l - 1) & 1, dst = l & 1;
-        if (tid < (WG >> l)) {
-            const uint32_t tau = tid >> (LOGL - l);
+        if (tid < (NL >> l)) {
+            const uint32_t t2 = tid >> (LOGL - l);
             const uint32_t x = tid & ((L >> l) - 1);
-            const uint32_t g2 = blockIdx.x * TPW + tau;
+            const uint32_t g2 = blockIdx.x * TPW + t2;
             const bool act2 = g2 < ntrees;
             const uint32_t req2 = act2 ? (MODE == 0 ? g2 / P::d : g2) : 0;
             const uint32_t lay2 = MODE == 0 ? (act2 ? g2 % P::d : 0) : P::d - 1;

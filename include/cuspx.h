@@ -40,6 +40,11 @@ int cuspx_num_params(void);
 int cuspx_param_id(const char *name);            /* "sha2-128f", ... ; -2 if unknown */
 int cuspx_get_param_info(int pid, cuspx_param_info *out);
 int cuspx_set_default_param(int pid);
+/* lanes per WOTS leaf (1, 2, 4, 8) of the hypertree kernel for set pid and
+ * op (0 sign, 2 keygen) at every batch size; 0 restores the tuning table /
+ * built-in choice.  cuspx_get_split reports the choice for a batch size. */
+int cuspx_set_split(int pid, int op, int lanes_per_leaf);
+int cuspx_get_split(int pid, int op, uint32_t num);
 int cuspx_get_default_param(void);
 void cuspx_set_debug_mode(int on);
 int cuspx_get_debug_mode(void);
@@ -166,6 +171,9 @@ int spx_server_wait(spx_server *s, uint64_t ticket, int timeout_ms, uint8_t *sig
 int spx_server_cancel(spx_server *s, uint64_t ticket);
 int spx_server_stats(spx_server *s, spx_server_statistics *st);
 int spx_server_reset_stats(spx_server *s);
+/* latency percentiles of one (parameter set, op) since the last reset */
+int spx_server_latency(spx_server *s, int pid, int op, float *p50_us, float *p99_us,
+                       uint64_t *count);
 int spx_server_destroy(spx_server *s);
 
 /* ------------------------------------------------------ device plumbing

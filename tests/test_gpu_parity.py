@@ -169,3 +169,26 @@ def test_batch_determinism_and_order_independence():
     perm = [3, 1, 7, 0, 5, 2, 6, 4]
     s3 = C.batch_sign(pid, sk[perm], [msgs[i] for i in perm], opt[perm])
     assert (s3 == s1[perm]).all()
+
+
+@pytest.mark.parametrize("name", ["sha2-128f", "sha2-128s", "shake256-128f", "sha2-256f",
+                                  "shake256-192f"])
+def test_every_leaf_split_matches_oracle(name):
+    pid = C.param_id(name)
+    p = O.params(pid)
+    num = 2
+    seeds, msgs, opts = _inputs(pid, num, 555 + pid, [45, 3])
+    want_pk = [O.seed_keypair(pid, s) for s in seeds]
+    want_sig = [O.signature_with_optrand(pid, msgs[i], want_pk[i][1], opts[i]) for i in range(num)]
+    try:
+        for S in (1, 2, 4, 8):
+            C.set_split(pid, C.OP_SIGN, S)
+            C.set_split(pid, C.OP_KEYGEN, S)
+            assert C.get_split(pid, C.OP_SIGN, num) == S
+            pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+            assert [bytes(x) for x in pk] == [w[0] for w in want_pk], S
+            sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+            assert [bytes(x) for x in sig] == want_sig, S
+    finally:
+        C.set_split(pid, C.OP_SIGN, 0)
+        C.set_split(pid, C.OP_KEYGEN, 0)
