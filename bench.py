@@ -178,6 +178,9 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, seed=5):
                     "verify_p50_ms": round(st["verify_p50_us"] / 1e3, 3),
                     "verify_p99_ms": round(st["verify_p99_us"] / 1e3, 3),
                     "mean_batch": round(st["mean_batch"], 2)})
+        for name in SERVE_SETS:
+            a, z, c = srv.latency(C.param_id(name), 0)
+            out["sign_ms_" + name] = [round(a / 1e3, 3), round(z / 1e3, 3), c]
         srv.reset_stats()
         reqs = mix(sat_requests)
         el, bad2 = run(srv, kids, reqs, [0.0] * len(reqs))

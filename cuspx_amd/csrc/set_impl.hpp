@@ -20,12 +20,16 @@ struct Layout {
     // lanes per WOTS leaf for a k_ht launch over `trees` subtrees: a tuned
     // value when the tuning table has one, else double S while the launch
     // would still fill less than half the chip (S = 1 for large batches)
+    // largest split whose subtree still fits one 1024-lane workgroup
+    static constexpr int max_split() {
+        return (1024 >> P::th) >= 8 ? 8 : ((1024 >> P::th) < 1 ? 1 : (1024 >> P::th));
+    }
     static int split(int pid, int op, uint32_t trees) {
         const int t = tuned_split(pid, op, trees);
-        if (t == 1 || t == 2 || t == 4 || t == 8) return t;
+        if (t == 1 || t == 2 || t == 4 || t == 8) return t < max_split() ? t : max_split();
         const uint64_t leaves = (uint64_t)trees << P::th;
         int S = 1;
-        while (S < 8 && leaves * S * 2 <= kFillLanes) S *= 2;
+        while (S < max_split() && leaves * S * 2 <= kFillLanes) S *= 2;
         return S;
     }
     template <int S>
@@ -96,8 +100,10 @@ struct SetImpl {
 
     template <int MODE, int S>
     static void launch_ht(const BatchArgs& A, uint32_t trees, hipStream_t s) {
-        hipLaunchKernelGGL((k_ht<P, MODE, S>), dim3(Lay::template ht_blocks<S>(trees)),
-                           dim3(HtShape<P, S>::WG), 0, s, A);
+        if constexpr (HtShape<P, S>::WG <= 1024) {
+            hipLaunchKernelGGL((k_ht<P, MODE, S>), dim3(Lay::template ht_blocks<S>(trees)),
+                               dim3(HtShape<P, S>::WG), 0, s, A);
+        }
     }
     template <int MODE>
     static void launch_ht(const BatchArgs& A, uint32_t trees, int S, hipStream_t s) {
