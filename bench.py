@@ -98,7 +98,8 @@ def make_inputs(C, pid, batch):
 SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
 
 
-def serve_phase(C, seconds, qps, sat_requests, single_requests, seed=5):
+def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=8, window_us=1000,
+                seed=5):
     """Serving workload: 256 resident keys over SHA2-128f / SHA2-128s /
     SHAKE256-256f, Poisson arrivals at `qps`, 80% sign / 20% verify, message
     sizes log-uniform in 16 B .. 64 KiB.  Returns latency percentiles at the
@@ -163,8 +164,8 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, seed=5):
         return el, bad
 
     out = {"sets": SERVE_SETS, "keys": len(keys), "mix": "80% sign / 20% verify",
-           "message_bytes": "log-uniform 16..65536"}
-    with C.Server(hbm_budget_bytes=24 << 30, slots=4, batch_window_us=2000) as srv:
+           "message_bytes": "log-uniform 16..65536", "slots": slots, "batch_window_us": window_us}
+    with C.Server(hbm_budget_bytes=32 << 30, slots=slots, batch_window_us=window_us) as srv:
         kids = [srv.add_key(pid, sk, pk) for pid, sk, pk in keys]
         n = int(qps * seconds)
         reqs = mix(n)
@@ -217,6 +218,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=5000.0)
     ap.add_argument("--serve-saturation", type=int, default=8000)
     ap.add_argument("--serve-single", type=int, default=150)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-window-us", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -312,7 +315,7 @@ def main():
         print("serving: %.0f s at %.0f req/s" % (args.serve_seconds, args.serve_qps),
               file=sys.stderr, flush=True)
         serve = serve_phase(C, args.serve_seconds, args.serve_qps, args.serve_saturation,
-                            args.serve_single)
+                            args.serve_single, args.serve_slots, args.serve_window_us)
     kt, kn = prof.get("k_ht", (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
     work = kernel_work(p, B)

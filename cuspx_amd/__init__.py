@@ -333,8 +333,8 @@ def keygen_device(pid, num, seed, seed_stride, pk, sk, ws, stream=None):
                   "cuspx_keygen_device")
 
 
-def profile_enable(on=True):
-    lib().cuspx_profile_enable(1 if on else 0)
+def profile_enable(on=True, by_set=False):
+    lib().cuspx_profile_enable((2 if by_set else 1) if on else 0)
 
 
 def profile_reset():

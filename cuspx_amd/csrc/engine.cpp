@@ -206,16 +206,18 @@ struct Prof {
         const char* name;
         hipEvent_t ev;
     };
+    std::vector<const char*> call_set;  // parameter set of each call
     std::vector<std::vector<Mark>> calls;
     std::map<hipStream_t, size_t> open;
     std::map<std::string, std::pair<double, int>> acc;
+    bool by_set = false;  // key the totals by "set:kernel"
 };
 static Prof& prof() {
     static Prof p;
     return p;
 }
 
-void prof_begin(hipStream_t s) {
+void prof_begin(hipStream_t s, const char* set) {
     Prof& p = prof();
     if (!p.on) return;
     std::lock_guard<std::mutex> lk(p.mu);
@@ -223,6 +225,7 @@ void prof_begin(hipStream_t s) {
     if (hipEventCreate(&e) != hipSuccess) return;
     (void)hipEventRecord(e, s);
     p.calls.push_back({{"", e}});
+    p.call_set.push_back(set);
     p.open[s] = p.calls.size() - 1;
 }
 
@@ -805,6 +808,7 @@ void cuspx_profile_enable(int on) {
     Prof& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     p.on = on != 0;
+    p.by_set = on == 2;
 }
 
 void cuspx_profile_reset(void) {
@@ -813,6 +817,7 @@ void cuspx_profile_reset(void) {
     for (auto& c : p.calls)
         for (auto& m : c) (void)hipEventDestroy(m.ev);
     p.calls.clear();
+    p.call_set.clear();
     p.open.clear();
     p.acc.clear();
 }
@@ -820,18 +825,21 @@ void cuspx_profile_reset(void) {
 int cuspx_profile_read(char* names, int names_cap, float* ms, int* count, int cap) {
     Prof& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
-    for (auto& c : p.calls) {
+    for (size_t ci = 0; ci < p.calls.size(); ci++) {
+        auto& c = p.calls[ci];
         for (size_t i = 0; i < c.size(); i++) (void)hipEventSynchronize(c[i].ev);
         for (size_t i = 1; i < c.size(); i++) {
             float t = 0.f;
             (void)hipEventElapsedTime(&t, c[i - 1].ev, c[i].ev);
-            auto& a = p.acc[c[i].name];
+            auto& a = p.acc[p.by_set ? std::string(p.call_set[ci]) + ":" + c[i].name
+                                     : std::string(c[i].name)];
             a.first += t;
             a.second += 1;
         }
         for (auto& m : c) (void)hipEventDestroy(m.ev);
     }
     p.calls.clear();
+    p.call_set.clear();
     p.open.clear();
     int k = 0;
     std::string all;

@@ -6,6 +6,6 @@
 #include <hip/hip_runtime.h>
 
 namespace spx {
-void prof_begin(hipStream_t s);              // start of one pipeline call
+void prof_begin(hipStream_t s, const char* set);  // start of one pipeline call
 void prof_mark(hipStream_t s, const char* kernel);  // after a kernel launch
 }  // namespace spx

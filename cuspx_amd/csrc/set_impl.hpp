@@ -96,6 +96,7 @@ struct SetImpl {
     using Lay = Layout<P>;
 
     static int pid_;
+    static const char* set_name_;
     static size_t workspace(uint32_t n, int op) { return (size_t)Lay::bytes(pid_, n, op); }
 
     template <int MODE, int S>
@@ -132,7 +133,7 @@ struct SetImpl {
         const int S = Lay::split(pid_, kOpSign, n * P::d);
         Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpSign, S);
         constexpr uint32_t fors_tpw = 256u >> (P::a - J);
-        prof_begin(s);
+        prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_prep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_prep");
         hipLaunchKernelGGL((k_fors<P, J>), dim3(cdiv((uint64_t)n * P::k, fors_tpw)), dim3(256), 0,
@@ -165,7 +166,7 @@ struct SetImpl {
         const uint32_t n = b.n;
         constexpr int kVerifyLanes = 16;  // lanes per request in k_vht
         constexpr uint32_t groups = 256 / kVerifyLanes;
-        prof_begin(s);
+        prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_vprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_vprep");
         hipLaunchKernelGGL((k_vfors<P>), dim3(cdiv((uint64_t)n * P::k, 64)), dim3(64), 0, s, A);
@@ -187,7 +188,7 @@ struct SetImpl {
         const uint32_t n = b.n;
         const int S = Lay::split(pid_, kOpKeygen, n);
         Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpKeygen, S);
-        prof_begin(s);
+        prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_kprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_kprep");
         launch_ht<1>(A, n, S, s);
@@ -209,6 +210,7 @@ struct SetImpl {
 
     static SetOps ops(const char* name, int pid) {
         pid_ = pid;
+        set_name_ = name;
         SetOps o{};
         o.name = name;
         o.pid = pid;
@@ -234,6 +236,8 @@ struct SetImpl {
 
 template <class P, int J>
 int SetImpl<P, J>::pid_ = -1;
+template <class P, int J>
+const char* SetImpl<P, J>::set_name_ = "";
 
 }  // namespace spx
 

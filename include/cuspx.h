@@ -198,9 +198,10 @@ int cuspx_device_info(int *compute_units, int *clock_khz, size_t *hbm_bytes, cha
                       int name_cap);
 
 /* ------------------------------------------------------ kernel timing hooks */
-/* When enabled, every launcher records a HIP event after each kernel on its
- * stream.  cuspx_profile_read synchronises, then reports per kernel name the
- * summed milliseconds and launch counts since the last reset. */
+/* When enabled (on = 1; on = 2 keys the totals by "set:kernel"), every
+ * launcher records a HIP event after each kernel on its stream.
+ * cuspx_profile_read synchronises, then reports per kernel name the summed
+ * milliseconds and launch counts since the last reset. */
 void cuspx_profile_enable(int on);
 int cuspx_profile_read(char *names, int names_cap, float *ms, int *count, int cap);
 void cuspx_profile_reset(void);
