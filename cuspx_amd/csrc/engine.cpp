@@ -808,7 +808,7 @@ void cuspx_profile_enable(int on) {
     Prof& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);
     p.on = on != 0;
-    p.by_set = on == 2;
+    if (on) p.by_set = on == 2;
 }
 
 void cuspx_profile_reset(void) {

@@ -4,7 +4,9 @@
 //   rows and each key's SHA-256 pub_seed midstate (computed once on the GPU
 //   when the key is added).
 // * Requests (sign or verify, any parameter set, message 0..max_msg bytes)
-//   queue per (set, op).  A scheduler thread keeps up to `slots` batches in
+//   queue per (set, op, message-size class: <= 2 KiB, <= 16 KiB, larger) so
+//   short messages never wait for the serial hash of a long one in the same
+//   launch.  A scheduler thread keeps up to `slots` batches in
 //   flight, each on its own stream with its own pinned staging and device
 //   arena.  When a slot frees, the oldest queue is packed into the next
 //   batch: immediately if the GPU is idle or the batch is full, otherwise
@@ -28,6 +30,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/cuspx.h"
@@ -88,7 +91,7 @@ struct Server {
     int err = 0;
     uint64_t next_ticket = 1;
     std::map<int, KeyTable> keys;
-    std::map<std::pair<int, int>, std::deque<ReqP>> queues;
+    std::map<std::tuple<int, int, int>, std::deque<ReqP>> queues;  // (set, op, size class)
     std::map<uint64_t, ReqP> live;
     std::vector<Slot> slots;
     hipStream_t setup = nullptr;
@@ -228,7 +231,7 @@ void Server::loop() {
             if (sl.busy) continue;
             // the queue whose head request is oldest
             std::deque<ReqP>* best = nullptr;
-            std::pair<int, int> bk;
+            std::tuple<int, int, int> bk;
             for (auto& kv : queues) {
                 if (kv.second.empty()) continue;
                 if (!best || kv.second.front()->t_submit < best->front()->t_submit) {
@@ -237,8 +240,9 @@ void Server::loop() {
                 }
             }
             if (!best) break;
-            const SetOps* o = get_set(bk.first);
-            const uint64_t per = item_bytes(o, bk.second);
+            const int b_pid = std::get<0>(bk), b_op = std::get<1>(bk);
+            const SetOps* o = get_set(b_pid);
+            const uint64_t per = item_bytes(o, b_op);
             // how many fit the batch limit and this slot's share of the budget
             uint32_t take = 0;
             uint64_t bytes = 4096, hbytes = 4096;
@@ -272,7 +276,7 @@ void Server::loop() {
                 reqs.push_back(r);
             }
             if (reqs.empty()) continue;
-            const int rc = launch(sl, bk.first, bk.second, reqs);
+            const int rc = launch(sl, b_pid, b_op, reqs);
             if (rc) {  // fail the batch, keep serving
                 for (auto& r : reqs) {
                     r->status = rc;
@@ -412,7 +416,8 @@ int spx_server_submit(spx_server* sv, int op, int32_t key_id, const uint8_t* msg
         r->ticket = s->next_ticket++;
         r->t_submit = Clock::now();
         s->pending_bytes += mlen + r->aux.size();
-        s->queues[{pid, op}].push_back(r);
+        const int cls = mlen <= 2048 ? 0 : (mlen <= 16384 ? 1 : 2);
+        s->queues[std::make_tuple(pid, op, cls)].push_back(r);
         s->live[r->ticket] = r;
         *ticket = r->ticket;
     }

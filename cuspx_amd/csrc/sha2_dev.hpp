@@ -226,30 +226,44 @@ struct Sha2 {
 
     // SHA-256 continuing from st over (pre[0..NPRE) || m[0..mlen)), where
     // `before` bytes were already absorbed.  m is 4-byte aligned; bytes past
-    // mlen are never read beyond the word holding the last byte.
+    // mlen are never read beyond the word holding the last byte.  Blocks made
+    // only of whole message words are loaded directly; the head (prefix) and
+    // the tail (last partial word, padding, length) go word by word.
     template <int NPRE>
     __device__ static void stream_msg(uint32_t st[8], const uint32_t pre[NPRE],
                                       const uint8_t* m, uint32_t mlen, uint32_t before) {
         const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
         const uint32_t ndata = NPRE + (mlen + 4) / 4;  // words holding data or 0x80
+        const uint32_t nfull = NPRE + mlen / 4;        // words made of message bytes only
         const uint32_t total = ((ndata + 2 + 15) / 16) * 16;
         const uint32_t* mw = reinterpret_cast<const uint32_t*>(m);
         uint32_t b[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) b[i] = 0;
-        for (uint32_t kk = 0; kk < total; kk++) {
+        uint32_t kk = 0;
+#pragma unroll 1
+        while (kk < total) {
+            if ((kk & 15) == 0 && kk >= (uint32_t)NPRE && kk + 16 <= nfull) {
+                const uint32_t* src = mw + (kk - NPRE);
+#pragma unroll
+                for (int i = 0; i < 16; i++) b[i] = bswap32(src[i]);
+                sha256_compress(st, b);
+                kk += 16;
+                continue;
+            }
             uint32_t x;
             if (kk < (uint32_t)NPRE) {
                 x = pre[0];
 #pragma unroll
                 for (int j = 1; j < NPRE; j++) x = (kk == (uint32_t)j) ? pre[j] : x;
             } else if (kk < ndata) {
-                uint32_t i = kk - NPRE;
-                int valid = (int)mlen - 4 * (int)i;
-                uint32_t raw = valid > 0 ? bswap32(mw[i]) : 0u;
-                if (valid >= 4) x = raw;
-                else {
-                    uint32_t keep = valid <= 0 ? 0u : (0xFFFFFFFFu << (8 * (4 - valid)));
+                const uint32_t i = kk - NPRE;
+                const int valid = (int)mlen - 4 * (int)i;
+                const uint32_t raw = valid > 0 ? bswap32(mw[i]) : 0u;
+                if (valid >= 4) {
+                    x = raw;
+                } else {
+                    const uint32_t keep = valid <= 0 ? 0u : (0xFFFFFFFFu << (8 * (4 - valid)));
                     x = (raw & keep) | (0x80000000u >> (8 * (valid < 0 ? 0 : valid)));
                 }
             } else if (kk == total - 2) {
@@ -261,6 +275,7 @@ struct Sha2 {
             }
             put16(b, (int)(kk & 15), x);
             if ((kk & 15) == 15) sha256_compress(st, b);
+            kk++;
         }
     }
 

@@ -245,6 +245,8 @@ struct Shake {
 
     // SHAKE256(pre || m) squeezed to OUTB bytes (OUTB <= 136).  m is 4-byte
     // aligned; the word holding the last message byte is the last one read.
+    // Whole 136-byte blocks of message words are xored in directly; the head
+    // after the prefix and the tail go word by word.
     template <int NPRE, int OUTB>
     __device__ static void stream_msg(const uint32_t pre[NPRE], const uint8_t* m, uint32_t mlen,
                                       uint8_t out[OUTB]) {
@@ -255,10 +257,19 @@ struct Shake {
         for (int i = 0; i < 50; i++) s[i] = i < NPRE ? pre[i] : 0u;
         int q = NPRE;
         const uint32_t full = mlen / 4, rem = mlen & 3;
+        uint32_t i = 0;
 #pragma unroll 1
-        for (uint32_t i = 0; i < full; i++) {
+        while (i < full) {
+            if (q == 0 && i + 34 <= full) {
+#pragma unroll
+                for (int w = 0; w < 34; w++) s[w] ^= mw[i + w];
+                keccak_f(s);
+                i += 34;
+                continue;
+            }
             xor_at(s, q, mw[i]);
             q++;
+            i++;
             if (q == 34) {
                 keccak_f(s);
                 q = 0;
@@ -270,7 +281,7 @@ struct Shake {
         s[33] ^= 0x80000000u;
         keccak_f(s);
 #pragma unroll
-        for (int i = 0; i < OUTB; i++) out[i] = (uint8_t)(s[i >> 2] >> (8 * (i & 3)));
+        for (int o = 0; o < OUTB; o++) out[o] = (uint8_t)(s[o >> 2] >> (8 * (o & 3)));
     }
 
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],

@@ -98,6 +98,9 @@ template <class P>
 __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
     using Hh = HashT<P>;
     constexpr int NW = P::nw;
+    // one serial hash chain per lane over the whole message: ask the SIMD
+    // arbiter to favour these waves over concurrent bulk hypertree waves
+    __builtin_amdgcn_s_setprio(3);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n_req) return;
     const uint32_t row = A.key_idx ? A.key_idx[i] : i;
@@ -138,6 +141,7 @@ template <class P>
 __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
     using Hh = HashT<P>;
     constexpr int NW = P::nw;
+    __builtin_amdgcn_s_setprio(3);  // serial message hash per lane, as in k_prep
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n_req) return;
     const uint32_t row = A.key_idx ? A.key_idx[i] : i;
