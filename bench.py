@@ -19,6 +19,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The serving phase keeps up to 8 batches in flight on their own streams; give
+# the process as many hardware queues, or streams share queues and serialise
+# (read by the HIP runtime when it initialises).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12   # MI355X: 256 CUs x 64 lanes x 2.4 GHz
 # VALU lane-ops per compression of the shipped k_ht: SQ_INSTS_VALU x 64 over the

@@ -830,7 +830,7 @@ int cuspx_profile_read(char* names, int names_cap, float* ms, int* count, int ca
         for (size_t i = 0; i < c.size(); i++) (void)hipEventSynchronize(c[i].ev);
         for (size_t i = 1; i < c.size(); i++) {
             float t = 0.f;
-            (void)hipEventElapsedTime(&t, c[i - 1].ev, c[i].ev);
+            if (hipEventElapsedTime(&t, c[i - 1].ev, c[i].ev) != hipSuccess) continue;
             auto& a = p.acc[p.by_set ? std::string(p.call_set[ci]) + ":" + c[i].name
                                      : std::string(c[i].name)];
             a.first += t;
@@ -852,6 +852,7 @@ int cuspx_profile_read(char* names, int names_cap, float* ms, int* count, int ca
         k++;
     }
     if (names && names_cap > 0) snprintf(names, names_cap, "%s", all.c_str());
+    (void)hipGetLastError();  // events of destroyed streams must not leak an error
     return k;
 }
 

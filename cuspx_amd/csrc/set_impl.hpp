@@ -133,6 +133,7 @@ struct SetImpl {
         const int S = Lay::split(pid_, kOpSign, n * P::d);
         Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpSign, S);
         constexpr uint32_t fors_tpw = 256u >> (P::a - J);
+        (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_prep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_prep");
@@ -164,8 +165,10 @@ struct SetImpl {
         A.n_req = b.n;
         Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpVerify, 1);
         const uint32_t n = b.n;
-        constexpr int kVerifyLanes = 16;  // lanes per request in k_vht
+        // lanes per request in k_vht: more for the long chains of n = 24, 32
+        constexpr int kVerifyLanes = P::len > 40 ? 32 : 16;
         constexpr uint32_t groups = 256 / kVerifyLanes;
+        (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_vprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_vprep");
@@ -188,6 +191,7 @@ struct SetImpl {
         const uint32_t n = b.n;
         const int S = Lay::split(pid_, kOpKeygen, n);
         Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpKeygen, S);
+        (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_kprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_kprep");
@@ -199,6 +203,7 @@ struct SetImpl {
     static int midstates(const uint8_t* pk, uint32_t pk_stride, uint32_t* mid, uint32_t n,
                          hipStream_t s) {
         if (n == 0) return 0;
+        (void)hipGetLastError();
         hipLaunchKernelGGL((k_midstate<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, pk, pk_stride, mid,
                            n);
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;

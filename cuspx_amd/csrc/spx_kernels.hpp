@@ -619,6 +619,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     constexpr int LEN = P::len;
     constexpr int R = 256 / G;                 // requests per workgroup
     constexpr int MAXC = (LEN + G - 1) / G;    // chains per lane
+    __builtin_amdgcn_s_setprio(2);  // d dependent layers per request: latency-bound
     __shared__ uint32_t pkbuf[R][LEN * NW];
     __shared__ uint32_t rootb[R][NW];
     __shared__ uint8_t steps[R][LEN];
