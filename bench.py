@@ -186,6 +186,9 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=8, window_
         for name in SERVE_SETS:
             a, z, c = srv.latency(C.param_id(name), 0)
             out["sign_ms_" + name] = [round(a / 1e3, 3), round(z / 1e3, 3), c]
+        for k, op in (("sign_queue_wait_ms", 0), ("sign_execution_ms", 1)):
+            a, z, c = srv.latency(-1, op)
+            out[k] = [round(a / 1e3, 3), round(z / 1e3, 3)]
         srv.reset_stats()
         reqs = mix(sat_requests)
         el, bad2 = run(srv, kids, reqs, [0.0] * len(reqs))

@@ -56,7 +56,7 @@ struct Request {
     int32_t status = 0;
     bool done = false;
     bool cancelled = false;
-    Clock::time_point t_submit, t_done;
+    Clock::time_point t_submit, t_launch, t_done;
 };
 using ReqP = std::shared_ptr<Request>;
 
@@ -99,6 +99,7 @@ struct Server {
     // statistics
     std::vector<float> lat_us[2];
     std::map<std::pair<int, int>, std::vector<float>> lat_set;
+    std::vector<float> wait_us, exec_us;  // sign: submit->launch, launch->done
     uint64_t batches = 0, batched_reqs = 0;
 
     // bytes one request of (pid, op) needs in a slot besides its message
@@ -174,6 +175,8 @@ int Server::launch(Slot& sl, int pid, int op, std::vector<ReqP>& reqs) {
         hipSuccess)
         return kErrHip;
     if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return kErrHip;
+    const auto tl = Clock::now();
+    for (auto& r : reqs) r->t_launch = tl;
     sl.busy = true;
     sl.pid = pid;
     sl.op = op;
@@ -203,6 +206,12 @@ void Server::harvest(Slot& sl) {
             if (lat_us[sl.op].size() < (1u << 24)) {
                 lat_us[sl.op].push_back(us);
                 lat_set[{sl.pid, sl.op}].push_back(us);
+                if (sl.op == 0) {
+                    wait_us.push_back(
+                        std::chrono::duration<float, std::micro>(r.t_launch - r.t_submit).count());
+                    exec_us.push_back(
+                        std::chrono::duration<float, std::micro>(now - r.t_launch).count());
+                }
             }
         } else {
             live.erase(r.ticket);
@@ -489,6 +498,8 @@ int spx_server_reset_stats(spx_server* sv) {
     s->lat_us[0].clear();
     s->lat_us[1].clear();
     s->lat_set.clear();
+    s->wait_us.clear();
+    s->exec_us.clear();
     s->batches = s->batched_reqs = 0;
     return kOk;
 }
@@ -498,6 +509,13 @@ int spx_server_latency(spx_server* sv, int pid, int op, float* p50_us, float* p9
     Server* s = (Server*)sv;
     if (!s || !p50_us || !p99_us || !count) return kErrArg;
     std::lock_guard<std::mutex> lk(s->mu);
+    if (pid < 0) {  // pid -1: sign queue wait (op 0) / sign execution (op 1)
+        const std::vector<float>& x = op == 0 ? s->wait_us : s->exec_us;
+        *p50_us = pct(x, 0.50);
+        *p99_us = pct(x, 0.99);
+        *count = x.size();
+        return kOk;
+    }
     auto it = s->lat_set.find({pid, op});
     if (it == s->lat_set.end()) {
         *p50_us = *p99_us = 0.f;

@@ -171,7 +171,9 @@ int spx_server_wait(spx_server *s, uint64_t ticket, int timeout_ms, uint8_t *sig
 int spx_server_cancel(spx_server *s, uint64_t ticket);
 int spx_server_stats(spx_server *s, spx_server_statistics *st);
 int spx_server_reset_stats(spx_server *s);
-/* latency percentiles of one (parameter set, op) since the last reset */
+/* latency percentiles of one (parameter set, op) since the last reset; with
+ * pid = -1, op 0 reports the sign queue wait (submit to launch) and op 1 the
+ * sign execution (launch to completion seen by the host) */
 int spx_server_latency(spx_server *s, int pid, int op, float *p50_us, float *p99_us,
                        uint64_t *count);
 int spx_server_destroy(spx_server *s);
