@@ -19,9 +19,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The serving phase keeps up to 8 batches in flight on their own streams; give
-# the process as many hardware queues, or streams share queues and serialise
-# (read by the HIP runtime when it initialises).
+# The serving phase keeps several batches in flight on their own streams; give
+# the process more hardware queues than streams, or streams share queues and
+# serialise (read by the HIP runtime when it initialises).
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12   # MI355X: 256 CUs x 64 lanes x 2.4 GHz
@@ -102,7 +102,7 @@ def make_inputs(C, pid, batch):
 SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
 
 
-def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=8, window_us=1000,
+def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_us=500,
                 seed=5):
     """Serving workload: 256 resident keys over SHA2-128f / SHA2-128s /
     SHAKE256-256f, Poisson arrivals at `qps`, 80% sign / 20% verify, message
@@ -222,11 +222,11 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
-    ap.add_argument("--serve-qps", type=float, default=5000.0)
+    ap.add_argument("--serve-qps", type=float, default=2500.0)
     ap.add_argument("--serve-saturation", type=int, default=8000)
     ap.add_argument("--serve-single", type=int, default=150)
-    ap.add_argument("--serve-slots", type=int, default=8)
-    ap.add_argument("--serve-window-us", type=int, default=1000)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-window-us", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
