@@ -211,6 +211,89 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_
     return out
 
 
+def mixed_phase(C, n, reps=3, seed=3):
+    """One mixed batch of n SHAKE256-128f / -192f signing requests (half
+    each, distinct DRBG keys, messages log-uniform 1 B .. 4 KiB) replayed as
+    one hipGraph; returns signs/s of the whole mixed batch."""
+    import random
+    import numpy as np
+    rng = random.Random(seed)
+    C.randombytes_init(bytes([seed]) * 48)
+    items = []
+    for name in ("shake256-128f", "shake256-192f"):
+        pid = C.param_id(name)
+        p = C.params(pid)
+        cnt = n // 2
+        pk, sk = C.batch_keygen(pid, np.frombuffer(C.randombytes(3 * p.n * cnt), np.uint8))
+        for i in range(cnt):
+            mlen = int(round(np.exp(rng.uniform(0, np.log(4096)))))
+            items.append((pid, bytes(sk[i]), rng.randbytes(mlen), C.randombytes(p.n)))
+    rng.shuffle(items)
+    mb = C.MixedSignBatch(items)
+    st = C.Stream()
+    mb.launch(st)
+    st.sync()
+    ts = []
+    for _ in range(reps):
+        a, z = C.Event(), C.Event()
+        a.record(st)
+        mb.launch(st)
+        z.record(st)
+        st.sync()
+        ts.append(a.elapsed_ms(z))
+    sigs = mb.signatures()
+    bad = 0
+    for k in range(0, len(items), max(1, len(items) // 16)):
+        pid, sk, m, _ = items[k]
+        p = C.params(pid)
+        pk = np.frombuffer(sk[2 * p.n:], np.uint8)
+        bad += int(C.batch_verify(pid, pk, np.frombuffer(sigs[k], np.uint8), [m])[0] != 0)
+    ms = sum(ts) / len(ts)
+    return {"sets": ["shake256-128f", "shake256-192f"], "requests": len(items),
+            "message_bytes": "log-uniform 1..4096", "ms_per_batch": round(ms, 3),
+            "signs_per_s": round(len(items) / ms * 1e3, 1), "spot_check_invalid": bad,
+            "launch": "one hipGraph per replay (per-set pipelines on forked streams)"}
+
+
+def graph_128s_phase(C, sizes=(1, 64, 1024), reps=3, seed=4):
+    """SHA2-128s signing batches of 1 / 64 / 1024 captured once as hipGraphs
+    and replayed; returns ms per batch and signs/s per size."""
+    import random
+    import numpy as np
+    pid = C.param_id("sha2-128s")
+    p = C.params(pid)
+    rng = random.Random(seed)
+    out = {}
+    for B in sizes:
+        pk, sk = C.batch_keygen(pid, np.frombuffer(rng.randbytes(3 * p.n * B), np.uint8))
+        D = C.DeviceBuffer
+        d_sk = D.from_array(sk)
+        d_msg = D.from_array(np.frombuffer(rng.randbytes(32 * B), np.uint8))
+        d_opt = D.from_array(np.frombuffer(rng.randbytes(p.n * B), np.uint8))
+        d_moff = D.from_array(np.arange(B, dtype=np.uint64) * 32)
+        d_mlen = D.from_array(np.full(B, 32, np.uint32))
+        d_sig = D(B * p.sig_bytes)
+        ws = D(C.workspace_bytes(pid, B, C.OP_SIGN))
+        g = C.graph_sign(pid, B, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr,
+                         d_opt.ptr, d_sig.ptr, p.sig_bytes, ws.ptr)
+        st = C.Stream()
+        g.launch(st)
+        st.sync()
+        ts = []
+        for _ in range(reps):
+            a, z = C.Event(), C.Event()
+            a.record(st)
+            g.launch(st)
+            z.record(st)
+            st.sync()
+            ts.append(a.elapsed_ms(z))
+        ms = sum(ts) / len(ts)
+        out[str(B)] = {"ms_per_batch": round(ms, 3), "signs_per_s": round(B / ms * 1e3, 1),
+                       "lanes_per_leaf": C.get_split(pid, C.OP_SIGN, B)}
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +308,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=2500.0)
     ap.add_argument("--serve-saturation", type=int, default=8000)
     ap.add_argument("--serve-single", type=int, default=150)
+    ap.add_argument("--mixed", type=int, default=8192)
+    ap.add_argument("--graph-128s", type=int, default=1)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-window-us", type=int, default=500)
     args = ap.parse_args()
@@ -317,6 +402,13 @@ def main():
                                   t["write_kib"] * t["write_scale"]))
     except (OSError, ValueError):
         pass
+    mixed = graph128s = None
+    if args.mixed > 0:
+        print("mixed SHAKE batch: %d requests" % args.mixed, file=sys.stderr, flush=True)
+        mixed = mixed_phase(C, args.mixed)
+    if args.graph_128s:
+        print("sha2-128s graph replay: batches 1/64/1024", file=sys.stderr, flush=True)
+        graph128s = graph_128s_phase(C)
     serve = None
     if args.serve_seconds > 0:
         print("serving: %.0f s at %.0f req/s" % (args.serve_seconds, args.serve_qps),
@@ -365,6 +457,8 @@ def main():
                      if (k_avg > 0 and p.hash == 0) else None},
         "cpu_baseline": cpu,
         "serve": serve,
+        "mixed_shake_batch": mixed,
+        "sha2_128s_graph": graph128s,
         "device": info,
     }
     print(json.dumps(out), flush=True)

@@ -545,3 +545,126 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+# ------------------------------------------------------- hipGraph replay
+
+class SignGroup(ctypes.Structure):
+    _fields_ = [("pid", ctypes.c_int), ("num", ctypes.c_uint32), ("sk", ctypes.c_void_p),
+                ("sk_stride", ctypes.c_uint32), ("msg", ctypes.c_void_p),
+                ("moff", ctypes.c_void_p), ("mlen", ctypes.c_void_p),
+                ("optrand", ctypes.c_void_p), ("sig", ctypes.c_void_p),
+                ("sig_stride", ctypes.c_uint64), ("ws", ctypes.c_void_p)]
+
+
+class Graph:
+    """An instantiated hipGraph of one batch's launch sequence."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def launch(self, stream=None):
+        L = lib()
+        L.cuspx_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _check(L.cuspx_graph_launch(self.h, stream.handle if stream else None),
+               "cuspx_graph_launch")
+
+    def close(self):
+        if self.h:
+            L = lib()
+            L.cuspx_graph_destroy.argtypes = [ctypes.c_void_p]
+            L.cuspx_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def graph_sign(pid, num, sk, sk_stride, msg, moff, mlen, optrand, sig, sig_stride, ws):
+    L = lib()
+    vp = ctypes.c_void_p
+    L.cuspx_graph_sign.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint32, vp, vp,
+                                   vp, vp, vp, ctypes.c_uint64, vp, ctypes.POINTER(vp)]
+    h = vp()
+    _check(L.cuspx_graph_sign(pid, num, sk, sk_stride, msg, moff, mlen, optrand, sig,
+                              sig_stride, ws, ctypes.byref(h)), "cuspx_graph_sign")
+    return Graph(h.value)
+
+
+def graph_verify(pid, num, pk, pk_stride, sig, sig_stride, msg, moff, mlen, status, ws):
+    L = lib()
+    vp = ctypes.c_void_p
+    L.cuspx_graph_verify.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
+                                     ctypes.c_uint64, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    h = vp()
+    _check(L.cuspx_graph_verify(pid, num, pk, pk_stride, sig, sig_stride, msg, moff, mlen,
+                                status, ws, ctypes.byref(h)), "cuspx_graph_verify")
+    return Graph(h.value)
+
+
+def graph_sign_mixed(groups):
+    """groups: list of SignGroup; one graph launch signs every group."""
+    L = lib()
+    vp = ctypes.c_void_p
+    L.cuspx_graph_sign_mixed.argtypes = [ctypes.POINTER(SignGroup), ctypes.c_int,
+                                         ctypes.POINTER(vp)]
+    arr = (SignGroup * len(groups))(*groups)
+    h = vp()
+    _check(L.cuspx_graph_sign_mixed(arr, len(groups), ctypes.byref(h)), "cuspx_graph_sign_mixed")
+    return Graph(h.value)
+
+
+class MixedSignBatch:
+    """A resident mixed-parameter-set signing batch: requests are grouped by
+    set, each group gets device buffers and a workspace, and the whole batch
+    is one hipGraph (one launch per replay, the sets run concurrently).
+
+    items: list of (pid, sk bytes, message bytes, optrand bytes)."""
+
+    def __init__(self, items):
+        self.groups = {}
+        for i, (pid, sk, m, opt) in enumerate(items):
+            self.groups.setdefault(pid, []).append((i, sk, m, opt))
+        self.n = len(items)
+        self.bufs = []
+        descs = []
+        self.layout = []
+        for pid, rows in sorted(self.groups.items()):
+            p = params(pid)
+            num = len(rows)
+            lens = [len(m) for _, _, m, _ in rows]
+            offs = np.zeros(num, np.uint64)
+            pos = 0
+            for j, ln in enumerate(lens):
+                offs[j] = pos
+                pos += (ln + 15) & ~15
+            msgbuf = np.zeros(pos + 16, np.uint8)
+            for j, (_, _, m, _) in enumerate(rows):
+                msgbuf[int(offs[j]):int(offs[j]) + len(m)] = np.frombuffer(m, np.uint8)
+            sk = np.frombuffer(b"".join(r[1] for r in rows), np.uint8)
+            opt = np.frombuffer(b"".join(r[3] for r in rows), np.uint8)
+            d = dict(sk=DeviceBuffer.from_array(sk), msg=DeviceBuffer.from_array(msgbuf),
+                     moff=DeviceBuffer.from_array(offs),
+                     mlen=DeviceBuffer.from_array(np.array(lens, np.uint32)),
+                     opt=DeviceBuffer.from_array(opt), sig=DeviceBuffer(num * p.sig_bytes),
+                     ws=DeviceBuffer(workspace_bytes(pid, num, OP_SIGN)))
+            self.bufs.append(d)
+            self.layout.append((pid, [r[0] for r in rows], p.sig_bytes))
+            descs.append(SignGroup(pid, num, d["sk"].ptr, p.sk_bytes, d["msg"].ptr, d["moff"].ptr,
+                                   d["mlen"].ptr, d["opt"].ptr, d["sig"].ptr, p.sig_bytes,
+                                   d["ws"].ptr))
+        self.graph = graph_sign_mixed(descs)
+
+    def launch(self, stream=None):
+        self.graph.launch(stream)
+
+    def signatures(self):
+        out = [None] * self.n
+        for d, (pid, idx, sb) in zip(self.bufs, self.layout):
+            s = d["sig"].download().reshape(len(idx), sb)
+            for j, i in enumerate(idx):
+                out[i] = bytes(s[j])
+        return out

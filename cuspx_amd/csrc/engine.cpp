@@ -731,6 +731,156 @@ int cuspx_keygen_device(int pid, uint32_t num, const uint8_t* seed, uint32_t see
     return o->keygen(b, pick_stream(stream));
 }
 
+struct cuspx_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+}  // extern "C"
+
+// capture fn(stream) (plus forked streams it records on) into a graph
+template <class Fn>
+static int capture(Fn fn, cuspx_graph** out) {
+    if (!out) return kErrArg;
+    hipStream_t cs;
+    HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    (void)hipGetLastError();
+    Prof& p = prof();
+    bool was_on;
+    {
+        std::lock_guard<std::mutex> lk(p.mu);
+        was_on = p.on;
+        p.on = false;  // no timing events inside a graph
+    }
+    int rc = kOk;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = fail(kErrHip, "hipStreamBeginCapture failed%s");
+    } else {
+        rc = fn(cs);
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (rc == kOk && e != hipSuccess) rc = fail(kErrHip, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    {
+        std::lock_guard<std::mutex> lk(p.mu);
+        p.on = was_on;
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    cuspx_graph* cg = new cuspx_graph();
+    cg->graph = g;
+    if (hipGraphInstantiate(&cg->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete cg;
+        return fail(kErrHip, "hipGraphInstantiate failed%s");
+    }
+    *out = cg;
+    return kOk;
+}
+
+extern "C" {
+
+int cuspx_graph_sign(int pid, uint32_t num, const uint8_t* sk, uint32_t sk_stride,
+                     const uint8_t* msg, const uint64_t* moff, const uint32_t* mlen,
+                     const uint8_t* optrand, uint8_t* sig, uint64_t sig_stride, void* ws,
+                     cuspx_graph** out) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    if (!num || !sk || !msg || !moff || !mlen || !optrand || !sig || !ws) return kErrArg;
+    DevBatch b{};
+    b.n = num;
+    b.sk = sk;
+    b.sk_stride = sk_stride;
+    b.msg = msg;
+    b.moff = moff;
+    b.mlen = mlen;
+    b.optrand = optrand;
+    b.sig = sig;
+    b.sig_stride = sig_stride;
+    b.ws = ws;
+    return capture([&](hipStream_t s) { return o->sign(b, s); }, out);
+}
+
+int cuspx_graph_verify(int pid, uint32_t num, const uint8_t* pk, uint32_t pk_stride,
+                       const uint8_t* sig, uint64_t sig_stride, const uint8_t* msg,
+                       const uint64_t* moff, const uint32_t* mlen, int32_t* status, void* ws,
+                       cuspx_graph** out) {
+    const SetOps* o = get_set(pid);
+    if (!o) return kErrParam;
+    if (!num || !pk || !sig || !msg || !moff || !mlen || !status || !ws) return kErrArg;
+    DevBatch b{};
+    b.n = num;
+    b.pk = pk;
+    b.pk_stride = pk_stride;
+    b.sig = const_cast<uint8_t*>(sig);
+    b.sig_stride = sig_stride;
+    b.msg = msg;
+    b.moff = moff;
+    b.mlen = mlen;
+    b.status = status;
+    b.ws = ws;
+    return capture([&](hipStream_t s) { return o->verify(b, s); }, out);
+}
+
+int cuspx_graph_sign_mixed(const cuspx_sign_group* groups, int ngroups, cuspx_graph** out) {
+    if (!groups || ngroups <= 0 || ngroups > 64) return kErrArg;
+    for (int i = 0; i < ngroups; i++)
+        if (!get_set(groups[i].pid)) return kErrParam;
+    std::vector<hipStream_t> fs(ngroups, nullptr);
+    std::vector<hipEvent_t> ev(ngroups + 1, nullptr);
+    for (int i = 0; i < ngroups; i++) HIP_OK(hipStreamCreateWithFlags(&fs[i], hipStreamNonBlocking));
+    for (auto& e : ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // fork: every group waits on the capture stream, runs on its own stream,
+    // and the capture stream joins all of them
+    int rc = capture(
+        [&](hipStream_t s) {
+            if (hipEventRecord(ev[ngroups], s) != hipSuccess) return (int)kErrHip;
+            for (int i = 0; i < ngroups; i++) {
+                const cuspx_sign_group& gr = groups[i];
+                if (hipStreamWaitEvent(fs[i], ev[ngroups], 0) != hipSuccess) return (int)kErrHip;
+                DevBatch b{};
+                b.n = gr.num;
+                b.sk = gr.sk;
+                b.sk_stride = gr.sk_stride;
+                b.msg = gr.msg;
+                b.moff = gr.moff;
+                b.mlen = gr.mlen;
+                b.optrand = gr.optrand;
+                b.sig = gr.sig;
+                b.sig_stride = gr.sig_stride;
+                b.ws = gr.ws;
+                if (gr.num) {
+                    const int r = get_set(gr.pid)->sign(b, fs[i]);
+                    if (r) return r;
+                }
+                if (hipEventRecord(ev[i], fs[i]) != hipSuccess) return (int)kErrHip;
+                if (hipStreamWaitEvent(s, ev[i], 0) != hipSuccess) return (int)kErrHip;
+            }
+            return (int)kOk;
+        },
+        out);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& f : fs) (void)hipStreamDestroy(f);
+    return rc;
+}
+
+int cuspx_graph_launch(cuspx_graph* g, void* stream) {
+    if (!g) return kErrArg;
+    HIP_OK(hipGraphLaunch(g->exec, pick_stream(stream)));
+    return kOk;
+}
+
+int cuspx_graph_destroy(cuspx_graph* g) {
+    if (!g) return kErrArg;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return kOk;
+}
+
 int cuspx_dev_alloc(void** ptr, size_t bytes) {
     if (!ptr) return kErrArg;
     HIP_OK(hipMalloc(ptr, bytes ? bytes : 1));

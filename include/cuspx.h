@@ -131,6 +131,39 @@ int cuspx_verify_device(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_st
 int cuspx_keygen_device(int pid, uint32_t num, const uint8_t *seed, uint32_t seed_stride,
                         uint8_t *pk, uint8_t *sk, void *ws, void *stream);
 
+/* ------------------------------------------------------------ hipGraph replay
+ * Capture the launch sequence of one device-resident batch (the arguments of
+ * cuspx_sign_device / cuspx_verify_device) into an instantiated hipGraph and
+ * replay it: every replay reads the same device buffers, so the caller
+ * refreshes their contents between replays.  No memset nodes are recorded. */
+typedef struct cuspx_graph cuspx_graph;
+int cuspx_graph_sign(int pid, uint32_t num, const uint8_t *sk, uint32_t sk_stride,
+                     const uint8_t *msg, const uint64_t *moff, const uint32_t *mlen,
+                     const uint8_t *optrand, uint8_t *sig, uint64_t sig_stride, void *ws,
+                     cuspx_graph **out);
+int cuspx_graph_verify(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_stride,
+                       const uint8_t *sig, uint64_t sig_stride, const uint8_t *msg,
+                       const uint64_t *moff, const uint32_t *mlen, int32_t *status, void *ws,
+                       cuspx_graph **out);
+/* a mixed batch: `groups` single-set sign batches, each on its own forked
+ * stream inside one graph (one launch signs every set concurrently) */
+typedef struct {
+    int pid;
+    uint32_t num;
+    const uint8_t *sk;
+    uint32_t sk_stride;
+    const uint8_t *msg;
+    const uint64_t *moff;
+    const uint32_t *mlen;
+    const uint8_t *optrand;
+    uint8_t *sig;
+    uint64_t sig_stride;
+    void *ws;
+} cuspx_sign_group;
+int cuspx_graph_sign_mixed(const cuspx_sign_group *groups, int ngroups, cuspx_graph **out);
+int cuspx_graph_launch(cuspx_graph *g, void *stream);
+int cuspx_graph_destroy(cuspx_graph *g);
+
 /* ------------------------------------------------------------ serving
  * Continuous-batching service: keys resident in HBM (sk / pk rows and the
  * SHA-256 pub_seed midstate per key), sign / verify requests of any
