@@ -1,0 +1,128 @@
+"""Autotuner for the hypertree kernel's lanes-per-leaf split.
+
+k_ht gives each WOTS leaf S lanes (S in 1, 2, 4, 8; each lane walks
+ceil(len/S) chains). A small batch fills the chip only with a wider split; a
+large one runs fastest at S=1, which needs no cross-lane reduction. The
+crossover depends on the set and on the batch, so it is measured: for every
+set, operation and batch size this times device-resident launches at each
+split the set allows and writes the winners as
+
+    <set> <sign|keygen> <min_subtrees> <lanes_per_leaf>
+
+lines (a sign launch covers batch*d subtrees, a keygen launch covers batch),
+the table libcuspx reads from tuning.txt beside itself or from $CUSPX_TUNING.
+
+    python -m cuspx_amd.autotune [--sets sha2-128f,...] [--out cuspx_amd/tuning.txt]
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+import cuspx_amd as C
+
+OPS = {"sign": 0, "keygen": 2}
+BATCHES_FAST = (1, 4, 16, 64, 256, 1024, 4096)
+BATCHES_SMALL = (1, 4, 16, 64, 256, 1024)
+
+
+def _time(fn, stream, reps):
+    fn()
+    stream.sync()
+    e0, e1 = C.Event(), C.Event()
+    best = float("inf")
+    for _ in range(reps):
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        stream.sync()
+        best = min(best, e0.elapsed_ms(e1))
+    return best
+
+
+def tune_set(name, batches, reps, log):
+    pid = C.param_id(name)
+    p = C.params(pid)
+    B = max(batches)
+    rng = np.random.default_rng(pid)
+    D = C.DeviceBuffer
+    stream = C.Stream()
+    sh = stream.handle
+    d_seed = D.from_array(rng.integers(0, 256, B * 3 * p.n, dtype=np.uint8))
+    d_opt = D.from_array(rng.integers(0, 256, B * p.n, dtype=np.uint8))
+    d_msg = D.from_array(rng.integers(0, 256, B * 32, dtype=np.uint8))
+    d_moff = D.from_array(np.arange(B, dtype=np.uint64) * 32)
+    d_mlen = D.from_array(np.full(B, 32, np.uint32))
+    d_pk, d_sk, d_sig = D(B * p.pk_bytes), D(B * p.sk_bytes), D(B * p.sig_bytes)
+    ws = D(max(C.workspace_bytes(pid, B, op) for op in (0, 1, 2)))
+    C.keygen_device(pid, B, d_seed.ptr, 3 * p.n, d_pk.ptr, d_sk.ptr, ws.ptr, sh)
+    stream.sync()
+    rows = []
+    for op_name, op in OPS.items():
+        for b in batches:
+            def run():
+                if op == 0:
+                    C.sign_device(pid, b, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr,
+                                  d_mlen.ptr, d_opt.ptr, d_sig.ptr, p.sig_bytes, ws.ptr, sh)
+                else:
+                    C.keygen_device(pid, b, d_seed.ptr, 3 * p.n, d_pk.ptr, d_sk.ptr, ws.ptr, sh)
+            times, seen = {}, set()
+            for S in (1, 2, 4, 8):
+                C.set_split(pid, op, S)
+                eff = C.get_split(pid, op, b)
+                if eff in seen:
+                    continue
+                seen.add(eff)
+                times[eff] = _time(run, stream, reps)
+            C.set_split(pid, op, 0)
+            default = C.get_split(pid, op, b)
+            win = min(times, key=times.get)
+            trees = b * p.d if op == 0 else b
+            rows.append((name, op_name, trees, win))
+            log("%-14s %-6s batch %5d  %s  -> S=%d (heuristic S=%d, %.3f vs %.3f ms)" % (
+                name, op_name, b, " ".join("S%d %.3f" % kv for kv in sorted(times.items())), win,
+                default, times[win], times[default]))
+    for buf in (d_seed, d_opt, d_msg, d_moff, d_mlen, d_pk, d_sk, d_sig, ws):
+        buf.free()
+    return rows
+
+
+def compact(rows):
+    """Keep only the lines where a set's choice changes as the batch grows."""
+    out, last = [], {}
+    for name, op, trees, S in sorted(rows, key=lambda r: (r[0], r[1], r[2])):
+        key = (name, op)
+        if last.get(key) != S:
+            out.append((name, op, 0 if key not in last else trees, S))
+            last[key] = S
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sets", default="all")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                  "tuning.txt"))
+    a = ap.parse_args(argv)
+    names = [C.params(i).name.decode() for i in range(12)] if a.sets == "all" else \
+        a.sets.split(",")
+    log = lambda s: print(s, flush=True)
+    rows, t0 = [], time.perf_counter()
+    for name in names:
+        small = name.endswith("s") and ("192" in name or "256" in name)
+        rows += tune_set(name, BATCHES_SMALL if small else BATCHES_FAST, a.reps, log)
+    with open(a.out, "w") as f:
+        f.write("# lanes-per-leaf split of k_ht: <set> <op> <min_subtrees> <lanes_per_leaf>\n")
+        f.write("# measured by python -m cuspx_amd.autotune on %s (%.0f s)\n"
+                % (C.device_info()["name"], time.perf_counter() - t0))
+        for r in compact(rows):
+            f.write("%s %s %d %d\n" % r)
+    log("wrote %s" % a.out)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
