@@ -24,13 +24,32 @@ sys.path.insert(0, ROOT)
 # serialise (read by the HIP runtime when it initialises).
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
-PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12   # MI355X: 256 CUs x 64 lanes x 2.4 GHz
-# VALU lane-ops per compression of the shipped k_ht: SQ_INSTS_VALU x 64 over the
-# compressions of one launch (profiles/k_ht_sha2_128f_b4096.md); the generic
-# FIPS 180-4 compression with v_alignbit / v_bitop3 / v_add3 is 64*14+48*10+8 = 1384
+# MI355X VALU issue rates (cuspx_amd/valu_rate, profiles/valu_issue_rates.md): the
+# simple ops (v_xor/v_add/v_lshrrev ..._e32, v_bitop3_b32) issue a wave64 instruction
+# every 2 cycles per SIMD, i.e. 32 lanes/clk -- the rate behind the published
+# 157.3 TFLOP/s fp32 vector peak; the other 3-operand ops (v_alignbit, v_add3,
+# v_perm, 64-bit shifts) every 4 cycles.
+CLK_HZ = 2.4e9
+SIMDS = 256 * 4
+PEAK_VALU_T = SIMDS * 32 * CLK_HZ / 1e12    # 78.64 T lane-instr/s (fast class)
+PEAK_VOP3_T = SIMDS * 16 * CLK_HZ / 1e12    # 39.32 T lane-instr/s (4-cycle class)
+# VALU lane-instructions per unit of the shipped k_ht: SQ_INSTS_VALU x 64 over the
+# units of one batch-4096 launch (profiles/k_ht_sha2_128f_b4096.md,
+# profiles/k_ht_shake256_128f_b4096.md)
 SHA256_OPS = 1232
+KECCAK_OPS = 4270
 SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
-KECCAK_OPS = 24 * 182                      # VALU ops of one Keccak-f[1600] on 32-bit halves
+# (4-cycle, 2-cycle) instructions per unit in the ISA of the built k_ht loop bodies:
+# F = 508 alignbit + 205 add3 + 4 perm | 312 bitop3 + 194 VOP2 add/shift;
+# Keccak-f = 24 x (58 alignbit | 120 bitop3 + 2 xor)
+SHA256_MIX = (717, 506)
+KECCAK_MIX = (24 * 58, 24 * 122)
+
+
+def mix_ceiling_units(mix):
+    """Units/s if every SIMD issued the unit's instruction mix back to back."""
+    slow, fast = mix
+    return SIMDS * 64 * CLK_HZ / (4 * slow + 2 * fast)
 
 
 def kernel_work(p, batch):
@@ -418,8 +437,10 @@ def main():
     kt, kn = prof.get("k_ht", (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
     work = kernel_work(p, B)
-    ops = work * (SHA256_OPS if p.hash == 0 else KECCAK_OPS)
-    achieved = ops / k_avg / 1e12 if k_avg > 0 else 0.0
+    ops_unit = SHA256_OPS if p.hash == 0 else KECCAK_OPS
+    units_s = work / k_avg if k_avg > 0 else 0.0
+    achieved = units_s * ops_unit / 1e12
+    ceiling = mix_ceiling_units(SHA256_MIX if p.hash == 0 else KECCAK_MIX)
     out = {
         "metric": "SHA2-128f signs/sec + verifies/sec on 1 GPU; p50/p99 sign latency at fixed QPS",
         "value": round(B / sign_s, 1),
@@ -445,16 +466,19 @@ def main():
         "verify_batch_ms": round(ver_s * 1e3, 3),
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
         "roofline": {"bound": "valu", "kernel": "k_ht", "achieved": round(achieved, 3),
-                     "peak": round(PEAK_INT_TOPS, 2), "unit": "TOP/s",
-                     "frac": round(achieved / PEAK_INT_TOPS, 4), "traffic": traffic,
+                     "peak": round(PEAK_VALU_T, 2), "unit": "T VALU lane-instr/s",
+                     "frac": round(achieved / PEAK_VALU_T, 4), "traffic": traffic,
+                     "peak_vop3": round(PEAK_VOP3_T, 2),
+                     "frac_of_vop3": round(achieved / PEAK_VOP3_T, 4),
+                     "mix_ceiling_units_per_s": round(ceiling, 1),
+                     "frac_of_mix_ceiling": round(units_s / ceiling, 4),
                      "work_per_launch": work,
                      "work_unit": "sha256 compressions" if p.hash == 0 else "keccak-f permutations",
-                     "ops_per_unit": SHA256_OPS if p.hash == 0 else KECCAK_OPS,
+                     "ops_per_unit": ops_unit,
                      "kernel_ms": round(k_avg * 1e3, 3),
-                     "units_per_s": round(work / k_avg, 1) if k_avg > 0 else 0.0,
-                     "frac_at_1450_ops": round(work / k_avg / (PEAK_INT_TOPS * 1e12 /
-                                                                SHA256_OPS_SURVEY), 4)
-                     if (k_avg > 0 and p.hash == 0) else None},
+                     "units_per_s": round(units_s, 1),
+                     "frac_at_1450_ops": round(units_s / (PEAK_VOP3_T * 1e12 / SHA256_OPS_SURVEY), 4)
+                     if p.hash == 0 else None},
         "cpu_baseline": cpu,
         "serve": serve,
         "mixed_shake_batch": mixed,

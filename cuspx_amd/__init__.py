@@ -24,7 +24,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcuspx.so")
+LIB_PATH = os.environ.get("CUSPX_LIBRARY") or os.path.join(_HERE, "libcuspx.so")
 
 OK, ERR_VERIFY, ERR_PARAM, ERR_HIP, ERR_ARG, ERR_BUDGET = 0, -1, -2, -3, -4, -5
 OP_SIGN, OP_VERIFY, OP_KEYGEN = 0, 1, 2

@@ -4,7 +4,8 @@
     python -m cuspx_amd.build --force    # rebuild everything
 
 Each parameter set is its own translation unit (csrc/set_*.hip) so the
-twelve instantiations compile in parallel.
+twelve instantiations compile in parallel. It also builds the VALU issue-rate
+probe (probe/valu_rate.hip -> cuspx_amd/valu_rate).
 """
 import concurrent.futures as cf
 import glob
@@ -16,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuspx.so")
+PROBE_SRC = os.path.join(HERE, "probe", "valu_rate.hip")
+PROBE = os.path.join(HERE, "valu_rate")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -67,6 +70,11 @@ def build(force=False, jobs=None, verbose=False):
         if p.returncode != 0:
             raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
         os.replace(tmp, LIB)
+    if force or not os.path.exists(PROBE) or os.path.getmtime(PROBE) < os.path.getmtime(PROBE_SRC):
+        cmd = [HIPCC] + FLAGS + ["-o", PROBE, PROBE_SRC]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("hipcc failed for valu_rate.hip:\n" + p.stdout + p.stderr)
     return LIB
 
 

@@ -950,7 +950,10 @@ int cuspx_device_info(int* compute_units, int* clock_khz, size_t* hbm_bytes, cha
     if (compute_units) *compute_units = pr.multiProcessorCount;
     if (clock_khz) *clock_khz = pr.clockRate;
     if (hbm_bytes) *hbm_bytes = pr.totalGlobalMem;
-    if (name && name_cap > 0) snprintf(name, name_cap, "%s (%s)", pr.name, pr.gcnArchName);
+    if (name && name_cap > 0) {
+        if (pr.name[0]) snprintf(name, name_cap, "%s (%s)", pr.name, pr.gcnArchName);
+        else snprintf(name, name_cap, "%s", pr.gcnArchName);
+    }
     return kOk;
 }
 

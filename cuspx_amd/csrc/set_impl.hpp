@@ -19,21 +19,25 @@ struct Layout {
 
     // lanes per WOTS leaf for a k_ht launch over `trees` subtrees: a tuned
     // value when the tuning table has one, else double S while the launch
-    // would still fill less than half the chip (S = 1 for large batches)
-    // largest split whose subtree still fits one 1024-lane workgroup
-    static constexpr int max_split() {
-        return (1024 >> P::th) >= 8 ? 8 : ((1024 >> P::th) < 1 ? 1 : (1024 >> P::th));
-    }
+    // would still fill less than half the chip (S = 1 for large batches).
+    // Subtrees too wide for one workgroup are split into parts, so every
+    // set takes S up to 8.
+    static constexpr int max_split() { return 8; }
     static int split(int pid, int op, uint32_t trees) {
         const int t = tuned_split(pid, op, trees);
-        if (t == 1 || t == 2 || t == 4 || t == 8) return t < max_split() ? t : max_split();
+        if (t == 1 || t == 2 || t == 4 || t == 8) return t;
         const uint64_t leaves = (uint64_t)trees << P::th;
         int S = 1;
         while (S < max_split() && leaves * S * 2 <= kFillLanes) S *= 2;
         return S;
     }
+    // most parts any split gives one subtree (1 when no split subtree exists)
+    static constexpr int max_parts() { return HtShape<P, 8>::PARTS; }
     template <int S>
-    static uint32_t ht_blocks(uint32_t trees) { return cdiv(trees, HtShape<P, S>::TPW); }
+    static uint32_t ht_blocks(uint32_t trees) {
+        return HtShape<P, S>::PARTS > 1 ? trees * HtShape<P, S>::PARTS
+                                        : cdiv(trees, HtShape<P, S>::TPW);
+    }
     template <int S>
     static uint64_t tl_cols(uint32_t trees) {
         return HtShape<P, S>::park ? (uint64_t)ht_blocks<S>(trees) * HtShape<P, S>::NL : 0;
@@ -60,6 +64,10 @@ struct Layout {
             uint64_t cols = 0;
             for (int S = 1; S <= 8; S *= 2) cols = tl_cols(trees, S) > cols ? tl_cols(trees, S) : cols;
             b += al(cols * P::len * P::nw * 4);
+            if (max_parts() > 1) {
+                b += al((uint64_t)trees * max_parts() * P::nw * 4);  // part roots
+                b += al((uint64_t)trees * 4);                       // part counters
+            }
         }
         (void)pid;
         return b + 256;
@@ -76,6 +84,8 @@ struct Layout {
         A.cap = nullptr;
         A.tl_buf = nullptr;
         A.tl_lanes = 0;
+        A.ht_parts = nullptr;
+        A.ht_count = nullptr;
         if (op == kOpSign) {
             A.roots = p;
             p += al((uint64_t)n * P::d * P::n);
@@ -86,6 +96,15 @@ struct Layout {
             const uint32_t trees = op == kOpSign ? n * P::d : n;
             A.tl_buf = (uint32_t*)p;
             A.tl_lanes = (uint32_t)tl_cols(trees, S);
+            uint64_t cols = 0;
+            for (int s2 = 1; s2 <= 8; s2 *= 2) cols = tl_cols(trees, s2) > cols ? tl_cols(trees, s2) : cols;
+            p += al(cols * P::len * P::nw * 4);
+            if (max_parts() > 1) {
+                A.ht_parts = (uint32_t*)p;
+                p += al((uint64_t)trees * max_parts() * P::nw * 4);
+                A.ht_count = (uint32_t*)p;
+                p += al((uint64_t)trees * 4);
+            }
         }
         (void)pid;
     }

@@ -12,6 +12,8 @@ namespace spx {
 
 enum HashKind { kSha2 = 0, kShake = 1 };
 
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
 // Compile-time parameter set (params.h:33-141).
 template <int HASH, int N_, int FULLH, int D_, int A_, int K_>
 struct Params {

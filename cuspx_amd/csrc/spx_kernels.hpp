@@ -60,6 +60,8 @@ struct BatchArgs {
     uint8_t* cap;             // [N][d][len][16][n]
     uint32_t* tl_buf;         // SHAKE T_len parking: [len*nw][lanes]
     uint32_t tl_lanes;
+    uint32_t* ht_parts;       // split subtrees: part roots [tree][part][nw]
+    uint32_t* ht_count;       // split subtrees: parts finished per tree (zeroed by the prep kernel)
     uint32_t n_req;
 };
 
@@ -134,6 +136,10 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
     plan_from_digest<P>(pl, dg);
     pl.status = 0;
     A.plan[i] = pl;
+    if (A.ht_count) {
+#pragma unroll 1
+        for (int layer = 0; layer < P::d; layer++) A.ht_count[(size_t)i * P::d + layer] = 0;
+    }
 }
 
 // verify: midstate, digest = H_msg(R from the signature), plan
@@ -210,6 +216,7 @@ __global__ void __launch_bounds__(64) k_kprep(BatchArgs A) {
     pl.leaf = 0;
     pl.status = 0;
     A.plan[i] = pl;
+    if (A.ht_count) A.ht_count[i] = 0;
     // sk = SK_SEED || SK_PRF || PUB_SEED (|| root, written by k_ht); pk = PUB_SEED || root
     uint8_t* sko = A.sk_out + (size_t)i * P::sk_bytes;
     uint8_t* pko = A.pk_out + (size_t)i * P::pk_bytes;
@@ -425,6 +432,44 @@ __device__ __forceinline__ uint64_t layer_tree(uint64_t tree, uint32_t layer) {
     return sh >= 64 ? 0ull : (tree >> sh);
 }
 
+// One Merkle node of a hypertree subtree (treehash, utils.cu:189-241): node x
+// of level l = H(left, right) under the subtree's HASHTREE address; writes the
+// node into the signature's auth path when it is the sibling of the signing
+// leaf's ancestor, and the subtree root (sign: roots[], keygen: pk / sk).
+template <class P, int MODE>
+__device__ __forceinline__ void ht_node(const BatchArgs& A, uint32_t g, uint32_t ntrees, int l,
+                                        uint32_t x, const uint32_t lft[P::nw],
+                                        const uint32_t rgt[P::nw], uint32_t out[P::nw]) {
+    using Hh = HashT<P>;
+    constexpr int NW = P::nw;
+    constexpr int L = 1 << P::th;
+    const bool act = g < ntrees;
+    const uint32_t req = act ? (MODE == 0 ? g / P::d : g) : 0;
+    const uint32_t lay = MODE == 0 ? (act ? g % P::d : 0) : P::d - 1;
+    const Plan& pl = A.plan[req];
+    typename Hh::Key key;
+    Hh::load_key(pl, key);
+    uint64_t tree = 0;
+    uint32_t target = 0;
+    if (MODE == 0) {
+        tree = layer_tree<P>(pl.tree, lay);
+        target = lay == 0 ? pl.leaf : (uint32_t)((pl.tree >> ((lay - 1) * P::th)) & (L - 1));
+    }
+    Adr ha = adr_make(lay, tree, kHashTree);
+    adr_node(ha, l, x);
+    Hh::H(key, ha, lft, rgt, out);
+    if (!act) return;
+    if (MODE == 0) {
+        uint8_t* a2 = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
+                      (size_t)lay * P::layer_bytes + P::wots_bytes;
+        if (l < P::th && x == ((target >> l) ^ 1u)) store_val<P::hash, NW>(a2 + (size_t)l * P::n, out);
+        if (l == P::th) store_val<P::hash, NW>(A.roots + ((size_t)req * P::d + lay) * P::n, out);
+    } else if (l == P::th) {
+        store_val<P::hash, NW>(A.sk_out + (size_t)req * P::sk_bytes + 3 * P::n, out);
+        store_val<P::hash, NW>(A.pk_out + (size_t)req * P::pk_bytes + P::n, out);
+    }
+}
+
 // Hypertree leaves.  MODE 0 = sign: every (request, layer) subtree, auth
 // path into the signature, layer root into roots[], chain values of the
 // signing leaf captured.  MODE 1 = keygen: the top subtree only, root into
@@ -435,13 +480,22 @@ __device__ __forceinline__ uint64_t layer_tree(uint64_t tree, uint32_t layer) {
 // of dependent compressions (latency of small batches); the launcher picks S.
 // With S > 1 (and always for SHAKE) the chain outputs are parked in tl_buf
 // ([word][leaf], coalesced) and lane q = 0 absorbs them into T_len.
+//
+// A subtree whose 2^h' x S lanes exceed one 1024-lane workgroup (the s sets
+// at S >= 4) is split over PARTS workgroups of 256 lanes.  Each part reduces
+// its leaves to a part root in LDS; the part that finishes last (device-scope
+// counter, release/acquire fences, no waiting) climbs the remaining log2(PARTS)
+// levels from the part roots.
 template <class P, int S>
 struct HtShape {
-    static constexpr int L = 1 << P::th;          // leaves per subtree
-    static constexpr int LT = L * S;              // lanes per subtree
-    static constexpr int WG = LT > 256 ? LT : 256;
-    static constexpr int TPW = WG / LT;           // subtrees per workgroup
-    static constexpr int NL = WG / S;             // leaves per workgroup
+    static constexpr int L = 1 << P::th;              // leaves per subtree
+    static constexpr int LT = L * S;                  // lanes per subtree
+    static constexpr int PARTS = LT > 1024 ? LT / 256 : 1;
+    static constexpr int LP = L / PARTS;              // leaves per part
+    static constexpr int LOGP = P::th - ilog2(PARTS); // levels reduced inside a part
+    static constexpr int WG = PARTS > 1 ? 256 : (LT > 256 ? LT : 256);
+    static constexpr int TPW = PARTS > 1 ? 1 : WG / LT;  // subtrees per workgroup
+    static constexpr int NL = WG / S;                 // leaves per workgroup
     static constexpr bool park = S > 1 || P::hash == kShake;
 };
 
@@ -452,15 +506,18 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
     using Sh = HtShape<P, S>;
     constexpr int NW = P::nw;
     constexpr int LOGL = P::th;
-    constexpr int L = Sh::L, LT = Sh::LT, TPW = Sh::TPW, NL = Sh::NL;
+    constexpr int L = Sh::L, TPW = Sh::TPW, NL = Sh::NL;
+    constexpr int PARTS = Sh::PARTS, LP = Sh::LP, LOGP = Sh::LOGP;
+    constexpr int LPT = PARTS > 1 ? LP * S : Sh::LT;  // lanes per subtree in this workgroup
     __shared__ uint32_t lds[2][NL][NW];
 
     const int tid = threadIdx.x;
-    const int tau = tid / LT;
-    const uint32_t u = (tid % LT) / S;  // leaf index = keypair address
-    const int q = tid % S;              // chain phase of this lane
-    const int slot = tau * L + u;       // leaf slot in the workgroup
-    const uint32_t g = blockIdx.x * TPW + tau;
+    const int tau = tid / LPT;
+    const uint32_t part = PARTS > 1 ? blockIdx.x % PARTS : 0;
+    const uint32_t u = part * LP + (tid % LPT) / S;  // leaf index = keypair address
+    const int q = tid % S;                            // chain phase of this lane
+    const int slot = tid / S;                         // leaf slot in the workgroup
+    const uint32_t g = (blockIdx.x / PARTS) * TPW + tau;
     const uint32_t ntrees = MODE == 0 ? A.n_req * P::d : A.n_req;
     const bool active = g < ntrees;
     const uint32_t req = active ? (MODE == 0 ? g / P::d : g) : 0;
@@ -531,50 +588,59 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
         for (int w = 0; w < NW; w++) lds[0][slot][w] = node[w];
     }
 
-    // ---- subtree: level l has NL >> l nodes, node x = H(2x, 2x+1)
+    // ---- subtree: level l has NL >> l nodes here, node x = H(2x, 2x+1)
 #pragma unroll 1
-    for (int l = 1; l <= LOGL; l++) {
+    for (int l = 1; l <= LOGP; l++) {
         __syncthreads();
         const int src = (l - 1) & 1, dst = l & 1;
         if (tid < (NL >> l)) {
-            const uint32_t t2 = tid >> (LOGL - l);
-            const uint32_t x = tid & ((L >> l) - 1);
-            const uint32_t g2 = blockIdx.x * TPW + t2;
-            const bool act2 = g2 < ntrees;
-            const uint32_t req2 = act2 ? (MODE == 0 ? g2 / P::d : g2) : 0;
-            const uint32_t lay2 = MODE == 0 ? (act2 ? g2 % P::d : 0) : P::d - 1;
-            const Plan& p2 = A.plan[req2];
-            Key k2;
-            Hh::load_key(p2, k2);
-            uint64_t tr2 = 0;
-            uint32_t tg2 = 0;
-            if (MODE == 0) {
-                tr2 = layer_tree<P>(p2.tree, lay2);
-                tg2 = lay2 == 0 ? p2.leaf
-                                : (uint32_t)((p2.tree >> ((lay2 - 1) * P::th)) & (L - 1));
-            }
-            uint32_t lft[NW], rgt[NW], out[NW];
+            const uint32_t t2 = tid >> (LOGP - l);
+            const uint32_t x = part * (LP >> l) + (tid & ((LP >> l) - 1));
+            uint32_t lft[NW], rgt[NW];
 #pragma unroll
             for (int w = 0; w < NW; w++) {
                 lft[w] = lds[src][2 * tid][w];
                 rgt[w] = lds[src][2 * tid + 1][w];
             }
-            Adr ha = adr_make(lay2, tr2, kHashTree);
-            adr_node(ha, l, x);
-            Hh::H(k2, ha, lft, rgt, out);
+            uint32_t out[NW];
+            ht_node<P, MODE>(A, (blockIdx.x / PARTS) * TPW + t2, ntrees, l, x, lft, rgt, out);
 #pragma unroll
             for (int w = 0; w < NW; w++) lds[dst][tid][w] = out[w];
-            if (act2) {
-                if (MODE == 0) {
-                    uint8_t* a2 = A.sig + (size_t)req2 * A.sig_stride + P::n + P::fors_bytes +
-                                  (size_t)lay2 * P::layer_bytes + P::wots_bytes;
-                    if (l < LOGL && x == ((tg2 >> l) ^ 1u))
-                        store_val<P::hash, NW>(a2 + (size_t)l * P::n, out);
-                    if (l == LOGL)
-                        store_val<P::hash, NW>(A.roots + ((size_t)req2 * P::d + lay2) * P::n, out);
-                } else if (l == LOGL) {
-                    store_val<P::hash, NW>(A.sk_out + (size_t)req2 * P::sk_bytes + 3 * P::n, out);
-                    store_val<P::hash, NW>(A.pk_out + (size_t)req2 * P::pk_bytes + P::n, out);
+        }
+    }
+    if constexpr (PARTS > 1) {
+        // part root -> ht_parts; the last part of the subtree climbs the rest
+        __shared__ uint32_t last;
+        if (tid == 0) {
+            uint32_t* dst = A.ht_parts + ((size_t)g * PARTS + part) * NW;
+#pragma unroll
+            for (int w = 0; w < NW; w++) dst[w] = lds[LOGP & 1][0][w];
+            __threadfence();  // release the part root before counting this part
+            last = active ? (atomicAdd(&A.ht_count[g], 1u) == (uint32_t)PARTS - 1) : 0u;
+        }
+        __syncthreads();
+        if (last) {
+            __threadfence();  // acquire: every part root is visible
+            if (tid < PARTS) {
+                const uint32_t* srcp = A.ht_parts + ((size_t)g * PARTS + tid) * NW;
+#pragma unroll
+                for (int w = 0; w < NW; w++)
+                    lds[LOGP & 1][tid][w] = __builtin_nontemporal_load(srcp + w);
+            }
+#pragma unroll 1
+            for (int l = LOGP + 1; l <= LOGL; l++) {
+                __syncthreads();
+                const int src = (l - 1) & 1, dst = l & 1;
+                if (tid < (L >> l)) {
+                    uint32_t lft[NW], rgt[NW], out[NW];
+#pragma unroll
+                    for (int w = 0; w < NW; w++) {
+                        lft[w] = lds[src][2 * tid][w];
+                        rgt[w] = lds[src][2 * tid + 1][w];
+                    }
+                    ht_node<P, MODE>(A, g, ntrees, l, tid, lft, rgt, out);
+#pragma unroll
+                    for (int w = 0; w < NW; w++) lds[dst][tid][w] = out[w];
                 }
             }
         }
