@@ -39,17 +39,19 @@ PEAK_VOP3_T = SIMDS * 16 * CLK_HZ / 1e12    # 39.32 T lane-instr/s (4-cycle clas
 SHA256_OPS = 1232
 KECCAK_OPS = 4270
 SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
-# (4-cycle, 2-cycle) instructions per unit in the ISA of the built k_ht loop bodies:
-# F = 508 alignbit + 205 add3 + 4 perm | 312 bitop3 + 194 VOP2 add/shift;
-# Keccak-f = 24 x (58 alignbit | 120 bitop3 + 2 xor)
-SHA256_MIX = (717, 506)
-KECCAK_MIX = (24 * 58, 24 * 122)
+# Issue rate of independent instruction blocks in the proportions of the unit's
+# loop body (wave-instructions per SIMD per ns, cuspx_amd/valu_rate "mix" rows):
+# SHA-256 F = 717 four-cycle (508 alignbit, 205 add3, 4 perm) : 506 two-cycle
+# (312 bitop3, 194 VOP2) ~ 7:5; Keccak-f round = 58 alignbit : 122 bitop3/xor ~ 1:2.
+# Mixing the two classes issues slower than the sum of their separate rates,
+# so this measured rate, not a 4c/2c model, is the ceiling of a unit.
+MIX_RATE_SHA256 = 0.6235
+MIX_RATE_KECCAK = 0.7253
 
 
-def mix_ceiling_units(mix):
+def mix_ceiling_units(rate, ops_per_unit):
     """Units/s if every SIMD issued the unit's instruction mix back to back."""
-    slow, fast = mix
-    return SIMDS * 64 * CLK_HZ / (4 * slow + 2 * fast)
+    return rate * 1e9 * SIMDS * 64 / ops_per_unit
 
 
 def kernel_work(p, batch):
@@ -440,7 +442,7 @@ def main():
     ops_unit = SHA256_OPS if p.hash == 0 else KECCAK_OPS
     units_s = work / k_avg if k_avg > 0 else 0.0
     achieved = units_s * ops_unit / 1e12
-    ceiling = mix_ceiling_units(SHA256_MIX if p.hash == 0 else KECCAK_MIX)
+    ceiling = mix_ceiling_units(MIX_RATE_SHA256 if p.hash == 0 else MIX_RATE_KECCAK, ops_unit)
     out = {
         "metric": "SHA2-128f signs/sec + verifies/sec on 1 GPU; p50/p99 sign latency at fixed QPS",
         "value": round(B / sign_s, 1),
