@@ -1,16 +1,19 @@
-"""Autotuner for the hypertree kernel's lanes-per-leaf split.
+"""Autotuner for the work splits of the hypertree and verify kernels.
 
 k_ht gives each WOTS leaf S lanes (S in 1, 2, 4, 8; each lane walks
 ceil(len/S) chains). A small batch fills the chip only with a wider split; a
-large one runs fastest at S=1, which needs no cross-lane reduction. The
-crossover depends on the set and on the batch, so it is measured: for every
-set, operation and batch size this times device-resident launches at each
-split the set allows and writes the winners as
+large one runs fastest at S=1, which needs no cross-lane reduction. k_vht
+gives each request G lanes (16 to 128): wider groups shorten each layer's
+chain phase, narrower ones pack more requests per wave. The crossovers depend
+on the set and on the batch, so they are measured: for every set, operation
+and batch size this times device-resident launches at each split the set
+allows and writes the winners as
 
-    <set> <sign|keygen> <min_subtrees> <lanes_per_leaf>
+    <set> <sign|keygen|verify> <min_units> <lanes>
 
-lines (a sign launch covers batch*d subtrees, a keygen launch covers batch),
-the table libcuspx reads from tuning.txt beside itself or from $CUSPX_TUNING.
+lines (a sign launch covers batch*d subtrees, keygen and verify launches
+cover batch), the table libcuspx reads from tuning.txt beside itself or from
+$CUSPX_TUNING.
 
     python -m cuspx_amd.autotune [--sets sha2-128f,...] [--out cuspx_amd/tuning.txt]
 """
@@ -23,7 +26,7 @@ import numpy as np
 
 import cuspx_amd as C
 
-OPS = {"sign": 0, "keygen": 2}
+OPS = {"sign": 0, "keygen": 2, "verify": 1}
 BATCHES_FAST = (1, 4, 16, 64, 256, 1024, 4096)
 BATCHES_SMALL = (1, 4, 16, 64, 256, 1024)
 
@@ -56,9 +59,15 @@ def tune_set(name, batches, reps, log):
     d_moff = D.from_array(np.arange(B, dtype=np.uint64) * 32)
     d_mlen = D.from_array(np.full(B, 32, np.uint32))
     d_pk, d_sk, d_sig = D(B * p.pk_bytes), D(B * p.sk_bytes), D(B * p.sig_bytes)
+    d_st = D(4 * B)
     ws = D(max(C.workspace_bytes(pid, B, op) for op in (0, 1, 2)))
     C.keygen_device(pid, B, d_seed.ptr, 3 * p.n, d_pk.ptr, d_sk.ptr, ws.ptr, sh)
+    C.sign_device(pid, B, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr, d_opt.ptr,
+                  d_sig.ptr, p.sig_bytes, ws.ptr, sh)
     stream.sync()
+    lo = 32 if p.len > 40 else 16
+    widths = {0: (1, 2, 4, 8), 2: (1, 2, 4, 8), 1: tuple(G for G in (16, 32, 64, 128)
+                                                        if lo <= G <= (128 if p.len > 64 else 64))}
     rows = []
     for op_name, op in OPS.items():
         for b in batches:
@@ -66,10 +75,13 @@ def tune_set(name, batches, reps, log):
                 if op == 0:
                     C.sign_device(pid, b, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr,
                                   d_mlen.ptr, d_opt.ptr, d_sig.ptr, p.sig_bytes, ws.ptr, sh)
+                elif op == 1:
+                    C.verify_device(pid, b, d_pk.ptr, p.pk_bytes, d_sig.ptr, p.sig_bytes,
+                                    d_msg.ptr, d_moff.ptr, d_mlen.ptr, d_st.ptr, ws.ptr, sh)
                 else:
                     C.keygen_device(pid, b, d_seed.ptr, 3 * p.n, d_pk.ptr, d_sk.ptr, ws.ptr, sh)
             times, seen = {}, set()
-            for S in (1, 2, 4, 8):
+            for S in widths[op]:
                 C.set_split(pid, op, S)
                 eff = C.get_split(pid, op, b)
                 if eff in seen:
@@ -79,12 +91,16 @@ def tune_set(name, batches, reps, log):
             C.set_split(pid, op, 0)
             default = C.get_split(pid, op, b)
             win = min(times, key=times.get)
+            if op == 1:
+                st = d_st.download(np.int32, b)
+                if np.count_nonzero(st):
+                    raise RuntimeError("%s: verify rejected a valid signature" % name)
             trees = b * p.d if op == 0 else b
             rows.append((name, op_name, trees, win))
             log("%-14s %-6s batch %5d  %s  -> S=%d (heuristic S=%d, %.3f vs %.3f ms)" % (
                 name, op_name, b, " ".join("S%d %.3f" % kv for kv in sorted(times.items())), win,
                 default, times[win], times[default]))
-    for buf in (d_seed, d_opt, d_msg, d_moff, d_mlen, d_pk, d_sk, d_sig, ws):
+    for buf in (d_seed, d_opt, d_msg, d_moff, d_mlen, d_pk, d_sk, d_sig, d_st, ws):
         buf.free()
     return rows
 
@@ -115,7 +131,8 @@ def main(argv=None):
         small = name.endswith("s") and ("192" in name or "256" in name)
         rows += tune_set(name, BATCHES_SMALL if small else BATCHES_FAST, a.reps, log)
     with open(a.out, "w") as f:
-        f.write("# lanes-per-leaf split of k_ht: <set> <op> <min_subtrees> <lanes_per_leaf>\n")
+        f.write("# k_ht lanes per leaf (sign, keygen; min_units = subtrees) and k_vht lanes per\n"
+                "# request (verify; min_units = requests): <set> <op> <min_units> <lanes>\n")
         f.write("# measured by python -m cuspx_amd.autotune on %s (%.0f s)\n"
                 % (C.device_info()["name"], time.perf_counter() - t0))
         for r in compact(rows):

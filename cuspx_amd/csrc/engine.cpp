@@ -55,10 +55,12 @@ const SetOps* get_set(int pid) {
 
 // ------------------------------------------------------------ tuning table
 //
-// Text file of `<set> <sign|keygen> <min_subtrees> <lanes_per_leaf>` lines
-// written by the autotuner (python -m cuspx_amd.autotune); read once from
-// $CUSPX_TUNING or tuning.txt next to libcuspx.so.  For a launch over T
-// subtrees the entry with the largest min_subtrees <= T applies.
+// Text file of `<set> <sign|keygen|verify> <min_units> <lanes>` lines written
+// by the autotuner (python -m cuspx_amd.autotune); read once from
+// $CUSPX_TUNING or tuning.txt next to libcuspx.so.  sign / keygen: lanes per
+// WOTS leaf of k_ht for a launch over T subtrees; verify: lanes per request
+// of k_vht for a batch of T requests.  The entry with the largest
+// min_units <= T applies.
 
 struct Tuning {
     std::mutex mu;
@@ -89,7 +91,9 @@ struct Tuning {
             int pid = -1;
             for (int i = 0; i < 12; i++)
                 if (name == get_set(i)->name) pid = i;
-            const int o = op == "sign" ? kOpSign : (op == "keygen" ? kOpKeygen : -1);
+            const int o = op == "sign" ? kOpSign
+                          : op == "keygen" ? kOpKeygen
+                          : op == "verify" ? kOpVerify : -1;
             if (pid < 0 || o < 0) continue;
             split[{pid, o}][min_trees] = S;
         }
@@ -473,10 +477,14 @@ int cuspx_get_param_info(int pid, cuspx_param_info* out) {
 
 int cuspx_set_split(int pid, int op, int lanes_per_leaf) {
     if (!get_set(pid)) return kErrParam;
-    if ((op != kOpSign && op != kOpKeygen) ||
-        (lanes_per_leaf != 0 && lanes_per_leaf != 1 && lanes_per_leaf != 2 &&
-         lanes_per_leaf != 4 && lanes_per_leaf != 8))
+    const bool pow2 = lanes_per_leaf > 0 && (lanes_per_leaf & (lanes_per_leaf - 1)) == 0;
+    if (op == kOpVerify) {
+        const int lo = get_set(pid)->len > 40 ? 32 : 16, hi = get_set(pid)->len > 64 ? 128 : 64;
+        if (lanes_per_leaf != 0 && !(pow2 && lanes_per_leaf >= lo && lanes_per_leaf <= hi))
+            return kErrArg;
+    } else if ((op != kOpSign && op != kOpKeygen) || (lanes_per_leaf != 0 && !(pow2 && lanes_per_leaf <= 8))) {
         return kErrArg;
+    }
     Tuning& t = tuning();
     std::lock_guard<std::mutex> lk(t.mu);
     if (lanes_per_leaf == 0) t.forced.erase({pid, op});

@@ -54,14 +54,16 @@ struct SetOps {
     // mid[i] = SHA-256 midstate of public key row i (zeros for SHAKE sets)
     int (*midstates)(const uint8_t* pk, uint32_t pk_stride, uint32_t* mid, uint32_t n,
                      hipStream_t);
-    // lanes per WOTS leaf k_ht would use for a batch of n requests
+    // lanes per WOTS leaf k_ht (sign / keygen) or lanes per request k_vht
+    // (verify) would use for a batch of n requests
     int (*split)(int op, uint32_t n);
 };
 
 const SetOps* get_set(int pid);
 
-// lanes per WOTS leaf chosen by the tuning table for a k_ht launch over
-// `trees` subtrees of set pid (op kOpSign / kOpKeygen); 0 = no entry
+// lanes chosen by the tuning table: per WOTS leaf for a k_ht launch over
+// `trees` subtrees (op kOpSign / kOpKeygen), per request for a k_vht launch
+// over `trees` requests (kOpVerify); 0 = no entry
 int tuned_split(int pid, int op, uint32_t trees);
 int num_sets();
 

@@ -31,6 +31,19 @@ struct Layout {
         while (S < max_split() && leaves * S * 2 <= kFillLanes) S *= 2;
         return S;
     }
+    // lanes per request in k_vht: a tuned value, else the base width (more
+    // lanes for the long chain lists of n = 24, 32) doubled while the launch
+    // would fill under an eighth of the chip.  Wider groups shorten each
+    // layer's chain phase (longest chain) at the cost of idle lanes.
+    static constexpr int vmin() { return P::len > 40 ? 32 : 16; }
+    static constexpr int vmax() { return P::len > 64 ? 128 : 64; }
+    static int vsplit(int pid, uint32_t n) {
+        const int t = tuned_split(pid, kOpVerify, n);
+        if (t >= vmin() && t <= vmax() && (t & (t - 1)) == 0) return t;
+        int G = vmin();
+        while (G < vmax() && (uint64_t)n * G * 8 <= kFillLanes) G *= 2;
+        return G;
+    }
     // most parts any split gives one subtree (1 when no split subtree exists)
     static constexpr int max_parts() { return HtShape<P, 8>::PARTS; }
     template <int S>
@@ -135,6 +148,11 @@ struct SetImpl {
         }
     }
 
+    template <int G>
+    static void launch_vht(const BatchArgs& A, uint32_t n, hipStream_t s) {
+        hipLaunchKernelGGL((k_vht<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
+    }
+
     static int sign(const DevBatch& b, hipStream_t s) {
         BatchArgs A{};
         A.sk = b.sk;
@@ -184,9 +202,7 @@ struct SetImpl {
         A.n_req = b.n;
         Lay::carve(pid_, A, (uint8_t*)b.ws, b.n, kOpVerify, 1);
         const uint32_t n = b.n;
-        // lanes per request in k_vht: more for the long chains of n = 24, 32
-        constexpr int kVerifyLanes = P::len > 40 ? 32 : 16;
-        constexpr uint32_t groups = 256 / kVerifyLanes;
+        const int G = Lay::vsplit(pid_, n);
         (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_vprep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
@@ -195,7 +211,13 @@ struct SetImpl {
         prof_mark(s, "k_vfors");
         hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
-        hipLaunchKernelGGL((k_vht<P, kVerifyLanes>), dim3(cdiv(n, groups)), dim3(256), 0, s, A);
+        if (G == Lay::vmin()) {
+            launch_vht<Lay::vmin()>(A, n, s);
+        } else if (G == 2 * Lay::vmin() || 4 * Lay::vmin() > Lay::vmax()) {
+            launch_vht<2 * Lay::vmin()>(A, n, s);
+        } else if constexpr (4 * Lay::vmin() <= Lay::vmax()) {
+            launch_vht<4 * Lay::vmin()>(A, n, s);
+        }
         prof_mark(s, "k_vht");
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;
     }
@@ -229,6 +251,7 @@ struct SetImpl {
     }
 
     static int split(int op, uint32_t n) {
+        if (op == kOpVerify) return Lay::vsplit(pid_, n);
         return Lay::split(pid_, op, op == kOpSign ? n * P::d : n);
     }
 

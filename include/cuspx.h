@@ -40,8 +40,10 @@ int cuspx_num_params(void);
 int cuspx_param_id(const char *name);            /* "sha2-128f", ... ; -2 if unknown */
 int cuspx_get_param_info(int pid, cuspx_param_info *out);
 int cuspx_set_default_param(int pid);
-/* lanes per WOTS leaf (1, 2, 4, 8) of the hypertree kernel for set pid and
- * op (0 sign, 2 keygen) at every batch size; 0 restores the tuning table /
+/* Work split of set pid at every batch size.  op 0 (sign) / 2 (keygen): lanes
+ * per WOTS leaf of the hypertree kernel (1, 2, 4, 8).  op 1 (verify): lanes
+ * per request of the verify kernel (16, 32, 64 when len <= 40; 32, 64 when
+ * len <= 64; 32, 64, 128 when len = 67).  0 restores the tuning table /
  * built-in choice.  cuspx_get_split reports the choice for a batch size. */
 int cuspx_set_split(int pid, int op, int lanes_per_leaf);
 int cuspx_get_split(int pid, int op, uint32_t num);

@@ -193,3 +193,37 @@ def test_every_leaf_split_matches_oracle(name):
     finally:
         C.set_split(pid, C.OP_SIGN, 0)
         C.set_split(pid, C.OP_KEYGEN, 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sha2-128f", "shake256-192f", "sha2-256f", "sha2-128s"])
+def test_every_verify_width(name):
+    """k_vht at every lanes-per-request width accepts the valid signatures and
+    rejects one tampered byte in each part of the hypertree section."""
+    pid = C.param_id(name)
+    p = O.params(pid)
+    num = 6
+    seeds, msgs, opts = _inputs(pid, num, 777 + pid, [32, 0, 100])
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    assert bytes(sig[0]) == O.signature_with_optrand(pid, msgs[0], bytes(sk[0]), opts[0])
+    th = p.full_h // p.d
+    ht = p.n + p.k * (p.a + 1) * p.n                 # first hypertree byte
+    layer_bytes = (p.len + th) * p.n
+    bad = sig.copy()
+    bad[1, ht + 5] ^= 1                              # layer 0, first WOTS chain
+    bad[2, ht + (p.len - 1) * p.n] ^= 2              # layer 0, last (checksum) chain
+    bad[3, ht + p.len * p.n + 1] ^= 4                # layer 0, auth path
+    bad[4, ht + (p.d - 1) * layer_bytes + 7] ^= 8    # top layer, WOTS
+    lo = 32 if p.len > 40 else 16
+    hi = 128 if p.len > 64 else 64
+    try:
+        G = lo
+        while G <= hi:
+            C.set_split(pid, C.OP_VERIFY, G)
+            assert C.get_split(pid, C.OP_VERIFY, num) == G
+            assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * num, G
+            assert list(C.batch_verify(pid, pk, bad, msgs)) == [0, -1, -1, -1, -1, 0], G
+            G *= 2
+    finally:
+        C.set_split(pid, C.OP_VERIFY, 0)
