@@ -477,12 +477,15 @@ int cuspx_get_param_info(int pid, cuspx_param_info* out) {
 
 int cuspx_set_split(int pid, int op, int lanes_per_leaf) {
     if (!get_set(pid)) return kErrParam;
-    const bool pow2 = lanes_per_leaf > 0 && (lanes_per_leaf & (lanes_per_leaf - 1)) == 0;
+    const SetOps* o = get_set(pid);
     if (op == kOpVerify) {
-        const int lo = get_set(pid)->len > 40 ? 32 : 16, hi = get_set(pid)->len > 64 ? 128 : 64;
-        if (lanes_per_leaf != 0 && !(pow2 && lanes_per_leaf >= lo && lanes_per_leaf <= hi))
-            return kErrArg;
-    } else if ((op != kOpSign && op != kOpKeygen) || (lanes_per_leaf != 0 && !(pow2 && lanes_per_leaf <= 8))) {
+        const int g = lanes_per_leaf >= kVhtLoop ? lanes_per_leaf - kVhtLoop : lanes_per_leaf;
+        const int lo = o->len > 40 ? 32 : 16;
+        const int hi = lanes_per_leaf >= kVhtLoop ? (o->hash == 0 ? 64 : 0) : (o->len > 64 ? 128 : 64);
+        if (lanes_per_leaf != 0 && !(g >= lo && g <= hi && (g & (g - 1)) == 0)) return kErrArg;
+    } else if ((op != kOpSign && op != kOpKeygen) ||
+               (lanes_per_leaf != 0 && lanes_per_leaf != 1 && lanes_per_leaf != 2 &&
+                lanes_per_leaf != 4 && lanes_per_leaf != 8)) {
         return kErrArg;
     }
     Tuning& t = tuning();

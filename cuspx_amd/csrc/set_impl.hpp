@@ -4,6 +4,7 @@
 #include "engine.hpp"
 #include "profile.hpp"
 #include "spx_kernels.hpp"
+#include "vht_sha2.hpp"
 
 namespace spx {
 
@@ -31,15 +32,24 @@ struct Layout {
         while (S < max_split() && leaves * S * 2 <= kFillLanes) S *= 2;
         return S;
     }
-    // lanes per request in k_vht: a tuned value, else the base width (more
-    // lanes for the long chain lists of n = 24, 32) doubled while the launch
-    // would fill under an eighth of the chip.  Wider groups shorten each
-    // layer's chain phase (longest chain) at the cost of idle lanes.
+    // Verify kernel and lanes per request.  G in [vmin, vmax] selects the
+    // layer-synchronous k_vht with G lanes; for the SHA-256 sets kVhtLoop + G
+    // (G in [vmin, 64]) selects k_vht_sha2, the one-compression loop that
+    // overlaps T_len with the chains.  A tuned value when the table has one,
+    // else k_vht with the base width (more lanes for the long chain lists of
+    // n = 24, 32) doubled while the launch would fill under an eighth of the
+    // chip.  Wider groups shorten each layer's chain phase at the cost of
+    // idle lanes.
     static constexpr int vmin() { return P::len > 40 ? 32 : 16; }
     static constexpr int vmax() { return P::len > 64 ? 128 : 64; }
+    static bool vvalid(int t) {
+        const int g = t >= kVhtLoop ? t - kVhtLoop : t;
+        if (g < vmin() || (g & (g - 1)) != 0) return false;
+        return t >= kVhtLoop ? (P::hash == kSha2 && g <= 64) : g <= vmax();
+    }
     static int vsplit(int pid, uint32_t n) {
         const int t = tuned_split(pid, kOpVerify, n);
-        if (t >= vmin() && t <= vmax() && (t & (t - 1)) == 0) return t;
+        if (vvalid(t)) return t;
         int G = vmin();
         while (G < vmax() && (uint64_t)n * G * 8 <= kFillLanes) G *= 2;
         return G;
@@ -152,6 +162,11 @@ struct SetImpl {
     static void launch_vht(const BatchArgs& A, uint32_t n, hipStream_t s) {
         hipLaunchKernelGGL((k_vht<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
     }
+    template <int G>
+    static void launch_vloop(const BatchArgs& A, uint32_t n, hipStream_t s) {
+        if constexpr (P::hash == kSha2 && G <= 64)
+            hipLaunchKernelGGL((k_vht_sha2<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
+    }
 
     static int sign(const DevBatch& b, hipStream_t s) {
         BatchArgs A{};
@@ -211,12 +226,18 @@ struct SetImpl {
         prof_mark(s, "k_vfors");
         hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
-        if (G == Lay::vmin()) {
-            launch_vht<Lay::vmin()>(A, n, s);
-        } else if (G == 2 * Lay::vmin() || 4 * Lay::vmin() > Lay::vmax()) {
-            launch_vht<2 * Lay::vmin()>(A, n, s);
-        } else if constexpr (4 * Lay::vmin() <= Lay::vmax()) {
-            launch_vht<4 * Lay::vmin()>(A, n, s);
+        constexpr int V0 = Lay::vmin();
+        if (G >= kVhtLoop) {
+            const int g = G - kVhtLoop;
+            if (g == V0) launch_vloop<V0>(A, n, s);
+            else if (g == 2 * V0) launch_vloop<2 * V0>(A, n, s);
+            else launch_vloop<4 * V0>(A, n, s);
+        } else if (G == V0) {
+            launch_vht<V0>(A, n, s);
+        } else if (G == 2 * V0 || 4 * V0 > Lay::vmax()) {
+            launch_vht<2 * V0>(A, n, s);
+        } else if constexpr (4 * V0 <= Lay::vmax()) {
+            launch_vht<4 * V0>(A, n, s);
         }
         prof_mark(s, "k_vht");
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;

@@ -93,14 +93,30 @@ struct Sha2 {
         for (int i = 0; i < NW; i++) k.sk[i] = p.skseed[i];
     }
 
+    // Word i of compression block blk of T over M payload words d after the
+    // 22-byte address (the payload sits at a 2-byte offset: one v_alignbit
+    // per word); blk and i may be runtime values.
+    template <int M>
+    __device__ static __forceinline__ uint32_t block_word(const Adr& a, const uint32_t d[M],
+                                                          int blk, int i) {
+        constexpr int P = 6 + M;                      // words up to and incl. the 0x80 byte
+        constexpr int B = (P + 2 + 15) / 16;          // compression blocks
+        constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+        const int g = 16 * blk + i;
+        if (g < 5) return a.w[g];
+        if (g == 5) return a.w[5] | (d[0] >> 16);
+        if (g < 5 + M) return __builtin_amdgcn_alignbit(d[g - 6], d[g - 5], 16);
+        if (g == 5 + M) return (d[M - 1] << 16) | 0x8000u;
+        if (g == 16 * B - 1) return bits;
+        return 0;
+    }
+
     // T over M payload words after the 22-byte address, all block indices
     // known at compile time (used for F, H and short T_l).
     template <int M>
     __device__ static __forceinline__ void thash_words(const Key& k, const Adr& a,
                                                        const uint32_t d[M], uint32_t out[NW]) {
-        constexpr int P = 6 + M;                      // words up to and incl. the 0x80 byte
-        constexpr int B = (P + 2 + 15) / 16;          // compression blocks
-        constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+        constexpr int B = (6 + M + 2 + 15) / 16;
         uint32_t st[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = k.mid[i];
@@ -108,17 +124,7 @@ struct Sha2 {
         for (int blk = 0; blk < B; blk++) {
             uint32_t w[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int g = 16 * blk + i;
-                uint32_t x;
-                if (g < 5) x = a.w[g];
-                else if (g == 5) x = a.w[5] | (d[0] >> 16);
-                else if (g < 5 + M) x = __builtin_amdgcn_alignbit(d[g - 6], d[g - 5], 16);
-                else if (g == 5 + M) x = (d[M - 1] << 16) | 0x8000u;
-                else if (g == 16 * B - 1) x = bits;
-                else x = 0;
-                w[i] = x;
-            }
+            for (int i = 0; i < 16; i++) w[i] = block_word<M>(a, d, blk, i);
             sha256_compress(st, w);
         }
 #pragma unroll

@@ -216,14 +216,14 @@ def test_every_verify_width(name):
     bad[3, ht + p.len * p.n + 1] ^= 4                # layer 0, auth path
     bad[4, ht + (p.d - 1) * layer_bytes + 7] ^= 8    # top layer, WOTS
     lo = 32 if p.len > 40 else 16
-    hi = 128 if p.len > 64 else 64
+    widths = [G for G in (16, 32, 64, 128) if lo <= G <= (128 if p.len > 64 else 64)]
+    if C.params(pid).hash == 0:  # the one-compression-loop kernel of the SHA-256 sets
+        widths += [1000 + G for G in (16, 32, 64) if G >= lo]
     try:
-        G = lo
-        while G <= hi:
+        for G in widths:
             C.set_split(pid, C.OP_VERIFY, G)
             assert C.get_split(pid, C.OP_VERIFY, num) == G
             assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * num, G
             assert list(C.batch_verify(pid, pk, bad, msgs)) == [0, -1, -1, -1, -1, 0], G
-            G *= 2
     finally:
         C.set_split(pid, C.OP_VERIFY, 0)
