@@ -34,9 +34,9 @@ SIMDS = 256 * 4
 PEAK_VALU_T = SIMDS * 32 * CLK_HZ / 1e12    # 78.64 T lane-instr/s (fast class)
 PEAK_VOP3_T = SIMDS * 16 * CLK_HZ / 1e12    # 39.32 T lane-instr/s (4-cycle class)
 # VALU lane-instructions per unit of the shipped k_ht: SQ_INSTS_VALU x 64 over the
-# units of one batch-4096 launch (profiles/k_ht_sha2_128f_b4096.md,
+# units of one batch-4096 launch (profiles/k_sign_fused_sha2_128f_b4096.md,
 # profiles/k_ht_shake256_128f_b4096.md)
-SHA256_OPS = 1232
+SHA256_OPS = 1237
 KECCAK_OPS = 4270
 SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
 # Issue rate of independent instruction blocks in the proportions of the unit's
@@ -55,7 +55,9 @@ def mix_ceiling_units(rate, ops_per_unit):
 
 
 def kernel_work(p, batch):
-    """Compressions (SHA-2) or permutations (SHAKE) of one k_ht sign launch."""
+    """Compressions (SHA-2) or permutations (SHAKE) of the hypertree tiles of
+    one sign launch: every layer's WOTS leaves (PRF + 15 F per chain, T_len)
+    and subtree nodes."""
     th = p.full_h // p.d
     leaves = batch * p.d * (1 << th)
     nodes = batch * p.d * ((1 << th) - 1)
@@ -65,6 +67,13 @@ def kernel_work(p, batch):
         return leaves * (p.len * 16 + tlen) + nodes * hblk
     tlen = (p.n + 32 + p.len * p.n) // 136 + 1
     return leaves * (p.len * 16 + tlen) + nodes
+
+
+def fors_work(p, batch):
+    """Compressions / permutations of the FORS tiles of one sign launch:
+    k trees of 2^a leaves (PRF + F each) and 2^a - 1 nodes."""
+    hblk = -(-(22 + 2 * p.n + 9) // 64) if p.hash == 0 else 1
+    return batch * p.k * ((1 << p.a) * 2 + ((1 << p.a) - 1) * hblk)
 
 
 def cpu_baseline(pid, seconds, procs):
@@ -414,10 +423,12 @@ def main():
     sign_s = sum(sign_ms) / len(sign_ms) / 1e3
     ver_s = sum(ver_ms) / len(ver_ms) / 1e3
 
+    # the dominant kernel: the fused FORS + hypertree launch, or k_ht alone
+    kname = "k_sign_fused" if "k_sign_fused" in prof else "k_ht"
     traffic = None
     try:
         tj = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))
-        t = tj.get("k_ht/%s/%d" % (p.name.decode(), B))
+        t = tj.get("%s/%s/%d" % (kname, p.name.decode(), B))
         if t:
             traffic = int(1024 * (t["fetch_kib"] * t["fetch_scale"] +
                                   t["write_kib"] * t["write_scale"]))
@@ -436,9 +447,9 @@ def main():
               file=sys.stderr, flush=True)
         serve = serve_phase(C, args.serve_seconds, args.serve_qps, args.serve_saturation,
                             args.serve_single, args.serve_slots, args.serve_window_us)
-    kt, kn = prof.get("k_ht", (0.0, 0))
+    kt, kn = prof.get(kname, (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
-    work = kernel_work(p, B)
+    work = kernel_work(p, B) + (fors_work(p, B) if kname == "k_sign_fused" else 0)
     ops_unit = SHA256_OPS if p.hash == 0 else KECCAK_OPS
     units_s = work / k_avg if k_avg > 0 else 0.0
     achieved = units_s * ops_unit / 1e12
@@ -467,7 +478,7 @@ def main():
         "sign_batch_ms": round(sign_s * 1e3, 3),
         "verify_batch_ms": round(ver_s * 1e3, 3),
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
-        "roofline": {"bound": "valu", "kernel": "k_ht", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3),
                      "peak": round(PEAK_VALU_T, 2), "unit": "T VALU lane-instr/s",
                      "frac": round(achieved / PEAK_VALU_T, 4), "traffic": traffic,
                      "peak_vop3": round(PEAK_VOP3_T, 2),

@@ -80,6 +80,7 @@ struct Layout {
         if (op == kOpSign) {
             b += al((uint64_t)n * P::d * P::n);                    // layer roots
             b += al((uint64_t)n * P::d * P::len * 16 * P::n);      // chain capture
+            b += al(4);                                            // fused-kernel tile counter
         }
         if (op != kOpVerify) {
             // sized for every split, so a workspace stays valid whichever S runs
@@ -109,11 +110,14 @@ struct Layout {
         A.tl_lanes = 0;
         A.ht_parts = nullptr;
         A.ht_count = nullptr;
+        A.tile_ctr = nullptr;
         if (op == kOpSign) {
             A.roots = p;
             p += al((uint64_t)n * P::d * P::n);
             A.cap = p;
             p += al((uint64_t)n * P::d * P::len * 16 * P::n);
+            A.tile_ctr = (uint32_t*)p;
+            p += al(4);
         }
         if (op != kOpVerify) {
             const uint32_t trees = op == kOpSign ? n * P::d : n;
@@ -148,6 +152,40 @@ struct SetImpl {
                                dim3(HtShape<P, S>::WG), 0, s, A);
         }
     }
+    // FORS and hypertree tiles in one persistent launch (when the split's
+    // hypertree tiles are 256-lane workgroups); false = not applicable
+    template <int S>
+    static bool launch_fused(const BatchArgs& A, uint32_t n, hipStream_t s) {
+        if constexpr (HtShape<P, S>::WG == 256) {
+            static const uint32_t resident = [] {
+                int per_cu = 0, cus = 0, dev = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &per_cu, reinterpret_cast<const void*>(&k_sign_fused<P, J, S>), 256, 0) !=
+                        hipSuccess ||
+                    hipGetDevice(&dev) != hipSuccess ||
+                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                        hipSuccess)
+                    return 256u;
+                return (uint32_t)((per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1));
+            }();
+            constexpr uint32_t fors_tpw = 256u >> (P::a - J);
+            const uint32_t n_ht = Lay::template ht_blocks<S>(n * P::d);
+            const uint32_t n_fors = cdiv((uint64_t)n * P::k, fors_tpw);
+            const uint32_t grid = n_ht + n_fors < resident ? n_ht + n_fors : resident;
+            hipLaunchKernelGGL((k_sign_fused<P, J, S>), dim3(grid), dim3(256), 0, s, A, n_ht, n_fors);
+            return true;
+        }
+        return false;
+    }
+    static bool launch_fused(const BatchArgs& A, uint32_t n, int S, hipStream_t s) {
+        switch (S) {
+            case 1: return launch_fused<1>(A, n, s);
+            case 2: return launch_fused<2>(A, n, s);
+            case 4: return launch_fused<4>(A, n, s);
+            default: return launch_fused<8>(A, n, s);
+        }
+    }
+
     template <int MODE>
     static void launch_ht(const BatchArgs& A, uint32_t trees, int S, hipStream_t s) {
         switch (S) {
@@ -189,13 +227,25 @@ struct SetImpl {
         prof_begin(s, set_name_);
         hipLaunchKernelGGL((k_prep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_prep");
-        hipLaunchKernelGGL((k_fors<P, J>), dim3(cdiv((uint64_t)n * P::k, fors_tpw)), dim3(256), 0,
-                           s, A);
-        prof_mark(s, "k_fors");
-        hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
-        prof_mark(s, "k_fors_pk");
-        launch_ht<0>(A, n * P::d, S, s);
-        prof_mark(s, "k_ht");
+        // one persistent launch for FORS + hypertree tiles: FORS then overlaps
+        // the hypertree instead of running before it.  SHAKE sets keep the
+        // two kernels when the hypertree alone fills the chip (the fused
+        // kernel's larger register footprint costs them ~2.5% there).
+        const bool fuse = P::hash == kSha2 ||
+                          (((uint64_t)n * P::d << P::th) * S < kFillLanes);
+        if (fuse && launch_fused(A, n, S, s)) {
+            prof_mark(s, "k_sign_fused");
+            hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+            prof_mark(s, "k_fors_pk");
+        } else {
+            hipLaunchKernelGGL((k_fors<P, J>), dim3(cdiv((uint64_t)n * P::k, fors_tpw)), dim3(256),
+                               0, s, A);
+            prof_mark(s, "k_fors");
+            hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+            prof_mark(s, "k_fors_pk");
+            launch_ht<0>(A, n * P::d, S, s);
+            prof_mark(s, "k_ht");
+        }
         hipLaunchKernelGGL((k_wots_gather<P>), dim3(cdiv((uint64_t)n * P::d * P::len, 256)),
                            dim3(256), 0, s, A);
         prof_mark(s, "k_wots_gather");

@@ -62,6 +62,7 @@ struct BatchArgs {
     uint32_t tl_lanes;
     uint32_t* ht_parts;       // split subtrees: part roots [tree][part][nw]
     uint32_t* ht_count;       // split subtrees: parts finished per tree (zeroed by the prep kernel)
+    uint32_t* tile_ctr;       // fused signing: next tile (zeroed by the prep kernel)
     uint32_t n_req;
 };
 
@@ -140,6 +141,7 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
 #pragma unroll 1
         for (int layer = 0; layer < P::d; layer++) A.ht_count[(size_t)i * P::d + layer] = 0;
     }
+    if (A.tile_ctr && i == 0) *A.tile_ctr = 0;
 }
 
 // verify: midstate, digest = H_msg(R from the signature), plan
@@ -226,9 +228,11 @@ __global__ void __launch_bounds__(64) k_kprep(BatchArgs A) {
 
 // ------------------------------------------------------------------ FORS
 
-// sign: FORS trees, L = 2^(a-J) lanes per tree, 2^J leaves per lane
+// sign: FORS trees, L = 2^(a-J) lanes per tree, 2^J leaves per lane; `blk`
+// is the workgroup's tile (TPW trees), so the same body serves k_fors and the
+// fused persistent signing kernel
 template <class P, int J>
-__global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
+__device__ __forceinline__ void fors_tile(const BatchArgs& A, uint32_t blk) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     constexpr int NW = P::nw;
@@ -240,7 +244,7 @@ __global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
     __shared__ uint32_t lds[2][WG][NW];
 
     const int tid = threadIdx.x;
-    const uint32_t g = blockIdx.x * TPW + tid / L;  // global tree id
+    const uint32_t g = blk * TPW + tid / L;  // global tree id
     const uint32_t u = tid % L;
     const bool active = g < A.n_req * P::k;
     const uint32_t req = active ? g / P::k : 0;
@@ -301,7 +305,7 @@ __global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
         if (tid < (WG >> l)) {
             const uint32_t tau = tid >> (LOGL - l);
             const uint32_t x = tid & ((L >> l) - 1);
-            const uint32_t g2 = blockIdx.x * TPW + tau;
+            const uint32_t g2 = blk * TPW + tau;
             const bool act2 = g2 < A.n_req * P::k;
             const uint32_t req2 = act2 ? g2 / P::k : 0, t2 = act2 ? g2 % P::k : 0;
             const Plan& p2 = A.plan[req2];
@@ -331,6 +335,11 @@ __global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
             }
         }
     }
+}
+
+template <class P, int J>
+__global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
+    fors_tile<P, J>(A, blockIdx.x);
 }
 
 // FORS public key: T_k over the k roots (fors.cu:142), lane per request
@@ -500,7 +509,7 @@ struct HtShape {
 };
 
 template <class P, int MODE, int S>
-__global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
+__device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     using Sh = HtShape<P, S>;
@@ -513,11 +522,11 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
 
     const int tid = threadIdx.x;
     const int tau = tid / LPT;
-    const uint32_t part = PARTS > 1 ? blockIdx.x % PARTS : 0;
+    const uint32_t part = PARTS > 1 ? blk % PARTS : 0;
     const uint32_t u = part * LP + (tid % LPT) / S;  // leaf index = keypair address
     const int q = tid % S;                            // chain phase of this lane
     const int slot = tid / S;                         // leaf slot in the workgroup
-    const uint32_t g = (blockIdx.x / PARTS) * TPW + tau;
+    const uint32_t g = (blk / PARTS) * TPW + tau;
     const uint32_t ntrees = MODE == 0 ? A.n_req * P::d : A.n_req;
     const bool active = g < ntrees;
     const uint32_t req = active ? (MODE == 0 ? g / P::d : g) : 0;
@@ -534,7 +543,7 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
     }
     const bool capture = MODE == 0 && active && u == target;
     uint8_t* cap = A.cap + ((size_t)req * P::d + layer) * P::len * 16 * P::n;
-    const uint32_t leaf_g = blockIdx.x * NL + slot;  // leaf column in tl_buf
+    const uint32_t leaf_g = blk * NL + slot;  // leaf column in tl_buf
 
     // ---- WOTS public key of leaf u (wots_gen_leaf, sign.cu:87-104)
     Adr wa = adr_make(layer, tree, kWots);
@@ -603,7 +612,7 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
                 rgt[w] = lds[src][2 * tid + 1][w];
             }
             uint32_t out[NW];
-            ht_node<P, MODE>(A, (blockIdx.x / PARTS) * TPW + t2, ntrees, l, x, lft, rgt, out);
+            ht_node<P, MODE>(A, (blk / PARTS) * TPW + t2, ntrees, l, x, lft, rgt, out);
 #pragma unroll
             for (int w = 0; w < NW; w++) lds[dst][tid][w] = out[w];
         }
@@ -644,6 +653,34 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
                 }
             }
         }
+    }
+}
+
+template <class P, int MODE, int S>
+__global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
+    ht_tile<P, MODE, S>(A, blockIdx.x);
+}
+
+// Signing as one persistent kernel: a grid of resident 256-lane workgroups
+// pulls tiles from a device counter (zeroed by k_prep) until the queue is
+// empty -- first the hypertree tiles (long), then the FORS tiles (short),
+// which fill the tail.  The tiles are independent given the plans, so no
+// workgroup ever waits on another; every workgroup exits once the counter
+// passes the last tile.
+template <class P, int J, int S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_sign_fused(BatchArgs A, uint32_t n_ht, uint32_t n_fors) {
+    static_assert(HtShape<P, S>::WG == 256, "hypertree tiles of 256 lanes");
+    __shared__ uint32_t tile;
+#pragma unroll 1
+    while (true) {
+        if (threadIdx.x == 0) tile = atomicAdd(A.tile_ctr, 1u);
+        __syncthreads();
+        const uint32_t t = tile;
+        __syncthreads();  // every lane has the tile (and left the previous one)
+        if (t >= n_ht + n_fors) break;
+        if (t < n_ht) ht_tile<P, 0, S>(A, t);
+        else fors_tile<P, J>(A, t - n_ht);
     }
 }
 
