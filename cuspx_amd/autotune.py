@@ -4,8 +4,8 @@ k_ht gives each WOTS leaf S lanes (S in 1, 2, 4, 8; each lane walks
 ceil(len/S) chains). A small batch fills the chip only with a wider split; a
 large one runs fastest at S=1, which needs no cross-lane reduction. k_vht
 gives each request G lanes (16 to 128): wider groups shorten each layer's
-chain phase, narrower ones pack more requests per wave; for the SHA-256 sets
-1000 + G selects k_vht_sha2, which overlaps T_len with the chains. The crossovers depend
+chain phase, narrower ones pack more requests per wave; 1000 + G selects
+k_vht_loop, which overlaps T_len with the chains. The crossovers depend
 on the set and on the batch, so they are measured: for every set, operation
 and batch size this times device-resident launches at each split the set
 allows and writes the winners as
@@ -69,8 +69,7 @@ def tune_set(name, batches, reps, log):
     lo = 32 if p.len > 40 else 16
     hi = 128 if p.len > 64 else 64
     vw = [G for G in (16, 32, 64, 128) if lo <= G <= hi]
-    if p.hash == 0:  # SHA-256 sets: also the one-compression-loop kernel (1000 + lanes)
-        vw += [1000 + G for G in (16, 32, 64) if G >= lo]
+    vw += [1000 + G for G in (16, 32, 64) if G >= lo]  # the one-hash-unit-loop kernel
     widths = {0: (1, 2, 4, 8), 2: (1, 2, 4, 8), 1: tuple(vw)}
     rows = []
     for op_name, op in OPS.items():

@@ -481,7 +481,7 @@ int cuspx_set_split(int pid, int op, int lanes_per_leaf) {
     if (op == kOpVerify) {
         const int g = lanes_per_leaf >= kVhtLoop ? lanes_per_leaf - kVhtLoop : lanes_per_leaf;
         const int lo = o->len > 40 ? 32 : 16;
-        const int hi = lanes_per_leaf >= kVhtLoop ? (o->hash == 0 ? 64 : 0) : (o->len > 64 ? 128 : 64);
+        const int hi = lanes_per_leaf >= kVhtLoop ? 64 : (o->len > 64 ? 128 : 64);
         if (lanes_per_leaf != 0 && !(g >= lo && g <= hi && (g & (g - 1)) == 0)) return kErrArg;
     } else if ((op != kOpSign && op != kOpKeygen) ||
                (lanes_per_leaf != 0 && lanes_per_leaf != 1 && lanes_per_leaf != 2 &&

@@ -4,7 +4,7 @@
 #include "engine.hpp"
 #include "profile.hpp"
 #include "spx_kernels.hpp"
-#include "vht_sha2.hpp"
+#include "vht_loop.hpp"
 
 namespace spx {
 
@@ -33,9 +33,9 @@ struct Layout {
         return S;
     }
     // Verify kernel and lanes per request.  G in [vmin, vmax] selects the
-    // layer-synchronous k_vht with G lanes; for the SHA-256 sets kVhtLoop + G
-    // (G in [vmin, 64]) selects k_vht_sha2, the one-compression loop that
-    // overlaps T_len with the chains.  A tuned value when the table has one,
+    // layer-synchronous k_vht with G lanes; kVhtLoop + G (G in [vmin, 64])
+    // selects k_vht_loop, the one-hash-unit loop that overlaps T_len with the
+    // chains.  A tuned value when the table has one,
     // else k_vht with the base width (more lanes for the long chain lists of
     // n = 24, 32) doubled while the launch would fill under an eighth of the
     // chip.  Wider groups shorten each layer's chain phase at the cost of
@@ -45,7 +45,7 @@ struct Layout {
     static bool vvalid(int t) {
         const int g = t >= kVhtLoop ? t - kVhtLoop : t;
         if (g < vmin() || (g & (g - 1)) != 0) return false;
-        return t >= kVhtLoop ? (P::hash == kSha2 && g <= 64) : g <= vmax();
+        return t >= kVhtLoop ? g <= 64 : g <= vmax();
     }
     static int vsplit(int pid, uint32_t n) {
         const int t = tuned_split(pid, kOpVerify, n);
@@ -202,8 +202,8 @@ struct SetImpl {
     }
     template <int G>
     static void launch_vloop(const BatchArgs& A, uint32_t n, hipStream_t s) {
-        if constexpr (P::hash == kSha2 && G <= 64)
-            hipLaunchKernelGGL((k_vht_sha2<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
+        if constexpr (G <= 64)
+            hipLaunchKernelGGL((k_vht_loop<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
     }
 
     static int sign(const DevBatch& b, hipStream_t s) {

@@ -196,7 +196,8 @@ def test_every_leaf_split_matches_oracle(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["sha2-128f", "shake256-192f", "sha2-256f", "sha2-128s"])
+@pytest.mark.parametrize("name", ["sha2-128f", "shake256-192f", "sha2-256f", "sha2-128s",
+                                  "shake256-128f", "shake256-256f"])
 def test_every_verify_width(name):
     """k_vht at every lanes-per-request width accepts the valid signatures and
     rejects one tampered byte in each part of the hypertree section."""
@@ -217,8 +218,7 @@ def test_every_verify_width(name):
     bad[4, ht + (p.d - 1) * layer_bytes + 7] ^= 8    # top layer, WOTS
     lo = 32 if p.len > 40 else 16
     widths = [G for G in (16, 32, 64, 128) if lo <= G <= (128 if p.len > 64 else 64)]
-    if C.params(pid).hash == 0:  # the one-compression-loop kernel of the SHA-256 sets
-        widths += [1000 + G for G in (16, 32, 64) if G >= lo]
+    widths += [1000 + G for G in (16, 32, 64) if G >= lo]  # the one-hash-unit-loop kernel
     try:
         for G in widths:
             C.set_split(pid, C.OP_VERIFY, G)
@@ -227,3 +227,30 @@ def test_every_verify_width(name):
             assert list(C.batch_verify(pid, pk, bad, msgs)) == [0, -1, -1, -1, -1, 0], G
     finally:
         C.set_split(pid, C.OP_VERIFY, 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,num,S", [("sha2-128f", 1600, 1), ("shake256-128f", 1600, 1),
+                                        ("sha2-128s", 300, 2)])
+def test_large_batch_launch_paths(name, num, S):
+    """Batches big enough for the full-chip launch shapes: the fused persistent
+    signing kernel (SHA-256), the separate FORS and hypertree kernels (SHAKE
+    at a full chip, 128s with 512/1024-lane subtrees).  A sample is compared
+    with the oracle byte for byte; every signature must verify."""
+    pid = C.param_id(name)
+    p = O.params(pid)
+    rng = random.Random(4242 + pid)
+    seeds = [rng.randbytes(3 * p.n) for _ in range(num)]
+    msgs = [rng.randbytes(rng.randrange(0, 200)) for _ in range(num)]
+    opts = [rng.randbytes(p.n) for _ in range(num)]
+    try:
+        C.set_split(pid, C.OP_SIGN, S)
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+        sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    finally:
+        C.set_split(pid, C.OP_SIGN, 0)
+    for i in (0, num // 2, num - 1):
+        opk, osk = O.seed_keypair(pid, seeds[i])
+        assert bytes(pk[i]) == opk and bytes(sk[i]) == osk, i
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, msgs[i], osk, opts[i]), i
+    assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * num
