@@ -10,13 +10,13 @@
 // lane G-1 (the absorber) hashes T_len block by block as soon as the chains a
 // block covers are done -- overlapping T_len with the chain phase -- and then
 // climbs the auth path.  Every lane runs the same loop body: pick a job,
-// assemble one 64-byte block and the chaining state for it, run ONE
+// assemble one block and the chaining state for it, run ONE
 // SHA-256 compression (SHAKE256: one Keccak-f permutation; the absorber's
 // T_len sponge state lives in LDS), fold the result back.  F steps, T_len
 // blocks and Merkle nodes therefore share a single hash unit in the code and
-// issue together instead of one after the other.  Each group moves to its next layer the
-// iteration after its absorber publishes the layer root in LDS, independently
-// of the other groups of the wave.  Lanes of one group talk only through LDS
+// issue together instead of one after the other.  Each group moves to its
+// next layer the iteration after its absorber publishes the layer root in
+// LDS, independently of the other groups of the wave.  Lanes of one group talk only through LDS
 // within one wave, so no workgroup barrier is needed inside the loop.
 #pragma once
 #include "spx_kernels.hpp"
@@ -225,28 +225,28 @@ __global__ void __launch_bounds__(256) k_vht_loop(BatchArgs A) {
             }
         }
         if constexpr (SHA2) {
-        // ---- assemble the chaining state and the block
+            // ---- assemble the chaining state and the block
             uint32_t sin[8], w[16];
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < 8; i++) sin[i] = (job == 2 || (job == 3 && hb > 0)) ? tl[i] : mid[i];
             if (job == 1) {
                 Adr wa = adr_make(layer, tree, kWots);
                 adr_keypair<TH>(wa, leaf);
                 adr_chain(wa, cj);
                 adr_hash(wa, s);
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 16; i++) w[i] = Hh::template block_word<NW>(wa, v, 0, i);
             } else if (job == 2) {
                 const uint32_t* pk = pkb[grp];
                 if (blk > 0 && 16 * blk + 15 < 5 + TLW) {  // a block of payload words only
                     const int base = 16 * blk - 6;
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 16; i++)
                         w[i] = __builtin_amdgcn_alignbit(pk[base + i], pk[base + i + 1], 16);
                 } else {
                     Adr pa = adr_make(layer, tree, kWotsPk);
                     adr_keypair<TH>(pa, leaf);
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 16; i++) {
                         const int m = 16 * blk + i;
                         uint32_t x;
@@ -264,53 +264,53 @@ __global__ void __launch_bounds__(256) k_vht_loop(BatchArgs A) {
                 adr_node(ha, h + 1, li >> 1);
                 uint32_t d[2 * NW];
                 const bool odd = li & 1u;
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NW; k++) {
                     d[k] = odd ? ap[k] : node[k];
                     d[NW + k] = odd ? node[k] : ap[k];
                 }
                 if (HB == 1 || hb == 0) {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 16; i++) w[i] = Hh::template block_word<2 * NW>(ha, d, 0, i);
                 } else {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 16; i++) w[i] = Hh::template block_word<2 * NW>(ha, d, 1, i);
                 }
             }
             // ---- the one compression of this iteration
             uint32_t out[8];
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < 8; i++) out[i] = sin[i];
             if (job != 0) sha256_compress(out, w);  // out = sin + rounds(sin, w)
             // ---- fold the result back
             if (job == 1) {
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NW; k++) v[k] = out[k];
                 s++;
                 settle();
             } else if (job == 2) {
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 8; i++) tl[i] = out[i];
                 blk++;
                 if (blk == TLB) {  // WOTS public key compressed: the leaf
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NW; k++) node[k] = tl[k];
                     phase = 1;
                     li = leaf;
                 }
             } else if (job == 3) {
                 if (hb + 1 < HB) {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 8; i++) tl[i] = out[i];
                     hb++;
                 } else {
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NW; k++) node[k] = out[k];
                     hb = 0;
                     li >>= 1;
                     h++;
                     if (h == TH) {  // layer root: publish, the group moves on next iteration
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NW; k++) rootb[grp][k] = node[k];
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                         epoch[grp] = layer + 1;
@@ -321,14 +321,14 @@ __global__ void __launch_bounds__(256) k_vht_loop(BatchArgs A) {
                 }
             }
         } else {
-        // ---- assemble the sponge state of this step (Keccak-f words, LE)
+            // ---- assemble the sponge state of this step (Keccak-f words, LE)
             uint32_t st[50];
             if (job == 1) {
                 Adr wa = adr_make(layer, tree, kWots);
                 adr_keypair<TH>(wa, leaf);
                 adr_chain(wa, cj);
                 adr_hash(wa, s);
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 50; i++) {
                     uint32_t x = 0;
                     if (i < NW) x = mid[i];
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(256) k_vht_loop(BatchArgs A) {
                 Adr pa = adr_make(layer, tree, kWotsPk);
                 adr_keypair<TH>(pa, leaf);
                 const uint32_t* pk = pkb[grp];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 50; i++) {
                     uint32_t x = sponge[grp][i];
                     if (i < 34) {
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(256) k_vht_loop(BatchArgs A) {
                 Adr ha = adr_make(layer, tree, kHashTree);
                 adr_node(ha, h + 1, li >> 1);
                 const bool odd = li & 1u;
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 50; i++) {
                     uint32_t x = 0;
                     if (i < NW) x = mid[i];
@@ -377,28 +377,28 @@ __global__ void __launch_bounds__(256) k_vht_loop(BatchArgs A) {
             if (job != 0) keccak_f(st);
             // ---- fold the result back
             if (job == 1) {
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NW; k++) v[k] = st[k];
                 s++;
                 settle();
             } else if (job == 2) {
                 blk++;
                 if (blk == TLB) {  // WOTS public key squeezed: the leaf
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NW; k++) node[k] = st[k];
                     phase = 1;
                     li = leaf;
                 } else {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 50; i++) sponge[grp][i] = st[i];
                 }
             } else if (job == 3) {
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NW; k++) node[k] = st[k];
                 li >>= 1;
                 h++;
                 if (h == TH) {  // layer root: publish, the group moves on next iteration
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NW; k++) rootb[grp][k] = node[k];
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     epoch[grp] = layer + 1;
