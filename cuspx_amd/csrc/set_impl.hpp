@@ -228,12 +228,8 @@ struct SetImpl {
         hipLaunchKernelGGL((k_prep<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_prep");
         // one persistent launch for FORS + hypertree tiles: FORS then overlaps
-        // the hypertree instead of running before it.  SHAKE sets keep the
-        // two kernels when the hypertree alone fills the chip (the fused
-        // kernel's larger register footprint costs them ~2.5% there).
-        const bool fuse = P::hash == kSha2 ||
-                          (((uint64_t)n * P::d << P::th) * S < kFillLanes);
-        if (fuse && launch_fused(A, n, S, s)) {
+        // the hypertree instead of running before it
+        if (launch_fused(A, n, S, s)) {
             prof_mark(s, "k_sign_fused");
             hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
             prof_mark(s, "k_fors_pk");

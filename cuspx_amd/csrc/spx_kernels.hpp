@@ -667,8 +667,10 @@ __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
 // which fill the tail.  The tiles are independent given the plans, so no
 // workgroup ever waits on another; every workgroup exits once the counter
 // passes the last tile.
+// SHA-256: 4 waves per SIMD, as k_ht; SHAKE256 (50-word sponge): 3, as k_ht --
+// forcing 4 there spills the permutation state
 template <class P, int J, int S>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::hash == kSha2 ? 4 : 3)))
 k_sign_fused(BatchArgs A, uint32_t n_ht, uint32_t n_fors) {
     static_assert(HtShape<P, S>::WG == 256, "hypertree tiles of 256 lanes");
     __shared__ uint32_t tile;
