@@ -35,9 +35,9 @@ PEAK_VALU_T = SIMDS * 32 * CLK_HZ / 1e12    # 78.64 T lane-instr/s (fast class)
 PEAK_VOP3_T = SIMDS * 16 * CLK_HZ / 1e12    # 39.32 T lane-instr/s (4-cycle class)
 # VALU lane-instructions per unit of the shipped k_ht: SQ_INSTS_VALU x 64 over the
 # units of one batch-4096 launch (profiles/k_sign_fused_sha2_128f_b4096.md,
-# profiles/k_ht_shake256_128f_b4096.md)
+# profiles/k_sign_fused_shake256_128f_b4096.md)
 SHA256_OPS = 1237
-KECCAK_OPS = 4270
+KECCAK_OPS = 4276
 SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
 # Issue rate of independent instruction blocks in the proportions of the unit's
 # loop body (wave-instructions per SIMD per ns, cuspx_amd/valu_rate "mix" rows):
