@@ -1,6 +1,6 @@
 """Autotuner for the work splits of the hypertree and verify kernels.
 
-k_ht gives each WOTS leaf S lanes (S in 1, 2, 4, 8; each lane walks
+k_ht gives each WOTS leaf S lanes (S in 1, 2, 4, 8, 16; each lane walks
 ceil(len/S) chains). A small batch fills the chip only with a wider split; a
 large one runs fastest at S=1, which needs no cross-lane reduction. k_vht
 gives each request G lanes (16 to 128): wider groups shorten each layer's
@@ -70,7 +70,7 @@ def tune_set(name, batches, reps, log):
     hi = 128 if p.len > 64 else 64
     vw = [G for G in (16, 32, 64, 128) if lo <= G <= hi]
     vw += [1000 + G for G in (16, 32, 64) if G >= lo]  # the one-hash-unit-loop kernel
-    widths = {0: (1, 2, 4, 8), 2: (1, 2, 4, 8), 1: tuple(vw)}
+    widths = {0: (1, 2, 4, 8, 16), 2: (1, 2, 4, 8, 16), 1: tuple(vw)}
     rows = []
     for op_name, op in OPS.items():
         for b in batches:

@@ -485,7 +485,7 @@ int cuspx_set_split(int pid, int op, int lanes_per_leaf) {
         if (lanes_per_leaf != 0 && !(g >= lo && g <= hi && (g & (g - 1)) == 0)) return kErrArg;
     } else if ((op != kOpSign && op != kOpKeygen) ||
                (lanes_per_leaf != 0 && lanes_per_leaf != 1 && lanes_per_leaf != 2 &&
-                lanes_per_leaf != 4 && lanes_per_leaf != 8)) {
+                lanes_per_leaf != 4 && lanes_per_leaf != 8 && lanes_per_leaf != 16)) {
         return kErrArg;
     }
     Tuning& t = tuning();

@@ -23,10 +23,10 @@ struct Layout {
     // would still fill less than half the chip (S = 1 for large batches).
     // Subtrees too wide for one workgroup are split into parts, so every
     // set takes S up to 8.
-    static constexpr int max_split() { return 8; }
+    static constexpr int max_split() { return 16; }
     static int split(int pid, int op, uint32_t trees) {
         const int t = tuned_split(pid, op, trees);
-        if (t == 1 || t == 2 || t == 4 || t == 8) return t;
+        if (t == 1 || t == 2 || t == 4 || t == 8 || t == 16) return t;
         const uint64_t leaves = (uint64_t)trees << P::th;
         int S = 1;
         while (S < max_split() && leaves * S * 2 <= kFillLanes) S *= 2;
@@ -55,7 +55,7 @@ struct Layout {
         return G;
     }
     // most parts any split gives one subtree (1 when no split subtree exists)
-    static constexpr int max_parts() { return HtShape<P, 8>::PARTS; }
+    static constexpr int max_parts() { return HtShape<P, 16>::PARTS; }
     template <int S>
     static uint32_t ht_blocks(uint32_t trees) {
         return HtShape<P, S>::PARTS > 1 ? trees * HtShape<P, S>::PARTS
@@ -70,7 +70,8 @@ struct Layout {
             case 1: return tl_cols<1>(trees);
             case 2: return tl_cols<2>(trees);
             case 4: return tl_cols<4>(trees);
-            default: return tl_cols<8>(trees);
+            case 8: return tl_cols<8>(trees);
+            default: return tl_cols<16>(trees);
         }
     }
     static uint64_t bytes(int pid, uint32_t n, int op) {
@@ -86,7 +87,7 @@ struct Layout {
             // sized for every split, so a workspace stays valid whichever S runs
             const uint32_t trees = op == kOpSign ? n * P::d : n;
             uint64_t cols = 0;
-            for (int S = 1; S <= 8; S *= 2) cols = tl_cols(trees, S) > cols ? tl_cols(trees, S) : cols;
+            for (int S = 1; S <= 16; S *= 2) cols = tl_cols(trees, S) > cols ? tl_cols(trees, S) : cols;
             b += al(cols * P::len * P::nw * 4);
             if (max_parts() > 1) {
                 b += al((uint64_t)trees * max_parts() * P::nw * 4);  // part roots
@@ -124,7 +125,7 @@ struct Layout {
             A.tl_buf = (uint32_t*)p;
             A.tl_lanes = (uint32_t)tl_cols(trees, S);
             uint64_t cols = 0;
-            for (int s2 = 1; s2 <= 8; s2 *= 2) cols = tl_cols(trees, s2) > cols ? tl_cols(trees, s2) : cols;
+            for (int s2 = 1; s2 <= 16; s2 *= 2) cols = tl_cols(trees, s2) > cols ? tl_cols(trees, s2) : cols;
             p += al(cols * P::len * P::nw * 4);
             if (max_parts() > 1) {
                 A.ht_parts = (uint32_t*)p;
@@ -182,7 +183,8 @@ struct SetImpl {
             case 1: return launch_fused<1>(A, n, s);
             case 2: return launch_fused<2>(A, n, s);
             case 4: return launch_fused<4>(A, n, s);
-            default: return launch_fused<8>(A, n, s);
+            case 8: return launch_fused<8>(A, n, s);
+            default: return launch_fused<16>(A, n, s);
         }
     }
 
@@ -192,7 +194,8 @@ struct SetImpl {
             case 1: launch_ht<MODE, 1>(A, trees, s); break;
             case 2: launch_ht<MODE, 2>(A, trees, s); break;
             case 4: launch_ht<MODE, 4>(A, trees, s); break;
-            default: launch_ht<MODE, 8>(A, trees, s); break;
+            case 8: launch_ht<MODE, 8>(A, trees, s); break;
+            default: launch_ht<MODE, 16>(A, trees, s); break;
         }
     }
 

@@ -518,7 +518,8 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     constexpr int L = Sh::L, TPW = Sh::TPW, NL = Sh::NL;
     constexpr int PARTS = Sh::PARTS, LP = Sh::LP, LOGP = Sh::LOGP;
     constexpr int LPT = PARTS > 1 ? LP * S : Sh::LT;  // lanes per subtree in this workgroup
-    __shared__ uint32_t lds[2][NL][NW];
+    constexpr int NLDS = NL > PARTS ? NL : PARTS;  // the last part climbs from PARTS roots
+    __shared__ uint32_t lds[2][NLDS][NW];
 
     const int tid = threadIdx.x;
     const int tau = tid / LPT;

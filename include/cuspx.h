@@ -41,7 +41,7 @@ int cuspx_param_id(const char *name);            /* "sha2-128f", ... ; -2 if unk
 int cuspx_get_param_info(int pid, cuspx_param_info *out);
 int cuspx_set_default_param(int pid);
 /* Work split of set pid at every batch size.  op 0 (sign) / 2 (keygen): lanes
- * per WOTS leaf of the hypertree kernel (1, 2, 4, 8).  op 1 (verify): lanes
+ * per WOTS leaf of the hypertree kernel (1, 2, 4, 8, 16).  op 1 (verify): lanes
  * per request of the layer-synchronous verify kernel (16, 32, 64 when
  * len <= 40; 32, 64 when len <= 64; 32, 64, 128 when len = 67); 1000 + lanes
  * (lanes 16 or 32 up to 64, as above) selects the verify kernel that runs one

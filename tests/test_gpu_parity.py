@@ -181,7 +181,7 @@ def test_every_leaf_split_matches_oracle(name):
     want_pk = [O.seed_keypair(pid, s) for s in seeds]
     want_sig = [O.signature_with_optrand(pid, msgs[i], want_pk[i][1], opts[i]) for i in range(num)]
     try:
-        for S in (1, 2, 4, 8):
+        for S in (1, 2, 4, 8, 16):
             C.set_split(pid, C.OP_SIGN, S)
             C.set_split(pid, C.OP_KEYGEN, S)
             # subtrees wider than one workgroup (s sets at S >= 4) run split in parts
