@@ -4,8 +4,8 @@
     python -m cuspx_amd.build --force    # rebuild everything
 
 Each parameter set is its own translation unit (csrc/set_*.hip) so the
-twelve instantiations compile in parallel. It also builds the VALU issue-rate
-probe (probe/valu_rate.hip -> cuspx_amd/valu_rate).
+twelve instantiations compile in parallel. It also builds the measurement
+probes (probe/valu_rate.hip, probe/chain_latency.hip -> cuspx_amd/<name>).
 """
 import concurrent.futures as cf
 import glob
@@ -17,8 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuspx.so")
-PROBE_SRC = os.path.join(HERE, "probe", "valu_rate.hip")
-PROBE = os.path.join(HERE, "valu_rate")
+# measurement probes: probe/<name>.hip -> cuspx_amd/<name>
+PROBES = ("valu_rate", "chain_latency")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -70,11 +70,14 @@ def build(force=False, jobs=None, verbose=False):
         if p.returncode != 0:
             raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
         os.replace(tmp, LIB)
-    if force or not os.path.exists(PROBE) or os.path.getmtime(PROBE) < os.path.getmtime(PROBE_SRC):
-        cmd = [HIPCC] + FLAGS + ["-o", PROBE, PROBE_SRC]
-        p = subprocess.run(cmd, capture_output=True, text=True)
-        if p.returncode != 0:
-            raise RuntimeError("hipcc failed for valu_rate.hip:\n" + p.stdout + p.stderr)
+    for name in PROBES:
+        src, exe = os.path.join(HERE, "probe", name + ".hip"), os.path.join(HERE, name)
+        if force or not os.path.exists(exe) or os.path.getmtime(exe) < max(
+                os.path.getmtime(src), _deps_mtime()):
+            cmd = [HIPCC] + FLAGS + ["-I" + CSRC, "-o", exe, src]
+            p = subprocess.run(cmd, capture_output=True, text=True)
+            if p.returncode != 0:
+                raise RuntimeError("hipcc failed for %s.hip:\n%s" % (name, p.stdout + p.stderr))
     return LIB
 
 
