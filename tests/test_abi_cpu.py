@@ -4,6 +4,8 @@ oracle's, and its host DRBG reproduces the NIST PQC KAT count-0 seed/msg
 (PQCgenKAT_sign.cu:55-63)."""
 import os
 import re
+import subprocess
+import sys
 
 import pytest
 
@@ -67,3 +69,63 @@ def test_workspace_sizes():
         for op in (C.OP_SIGN, C.OP_VERIFY, C.OP_KEYGEN):
             assert C.workspace_bytes(pid, 64, op) > 0
     assert C.workspace_bytes(99, 64, 0) == 0
+
+
+def test_workspace_grows_with_the_batch():
+    for pid in range(12):
+        for op in (C.OP_SIGN, C.OP_VERIFY, C.OP_KEYGEN):
+            sizes = [C.workspace_bytes(pid, n, op) for n in (1, 2, 7, 64, 1000, 4096)]
+            assert sizes == sorted(sizes), (pid, op)
+
+
+def test_split_validation_and_override():
+    """cuspx_set_split accepts exactly the documented values (host logic only)."""
+    L = C.lib()
+    for pid in range(12):
+        p = C.params(pid)
+        lo = 32 if p.len > 40 else 16
+        hi = 128 if p.len > 64 else 64
+        good = {C.OP_SIGN: [1, 2, 4, 8, 16], C.OP_KEYGEN: [1, 2, 4, 8, 16],
+                C.OP_VERIFY: [g for g in (16, 32, 64, 128) if lo <= g <= hi] +
+                [1000 + g for g in (16, 32, 64) if g >= lo]}
+        bad = {C.OP_SIGN: [3, 32, -1, 1016], C.OP_KEYGEN: [5, 64],
+               C.OP_VERIFY: [8, 1, 256, 1128, lo // 2, 1000 + lo // 2]}
+        try:
+            for op in good:
+                for v in good[op]:
+                    assert L.cuspx_set_split(pid, op, v) == C.OK, (pid, op, v)
+                    assert L.cuspx_get_split(pid, op, 5) == v, (pid, op, v)
+                for v in bad[op]:
+                    assert L.cuspx_set_split(pid, op, v) == C.ERR_ARG, (pid, op, v)
+        finally:
+            for op in good:
+                L.cuspx_set_split(pid, op, 0)
+    assert L.cuspx_set_split(99, 0, 1) == C.ERR_PARAM
+    assert L.cuspx_set_split(1, 7, 1) == C.ERR_ARG
+
+
+def test_tuning_table_is_read(tmp_path):
+    """A table given by $CUSPX_TUNING drives the automatic choice: the entry
+    with the largest threshold not above the launch size applies."""
+    table = tmp_path / "t.txt"
+    table.write_text("# test\nsha2-128f sign 0 4\nsha2-128f sign 2200 16\n"
+                     "sha2-128f verify 0 1032\nsha2-128f verify 50 16\nbogus line\n")
+    code = ("import cuspx_amd as C\n"
+            "L = C.lib()\n"
+            "print(L.cuspx_get_split(1, 0, 1), L.cuspx_get_split(1, 0, 99),"
+            " L.cuspx_get_split(1, 0, 100), L.cuspx_get_split(1, 1, 49), L.cuspx_get_split(1, 1, 50))\n")
+    env = dict(os.environ, CUSPX_TUNING=str(table))
+    out = subprocess.run([sys.executable, "-c", code], env=env, cwd=ROOT, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    # sign thresholds count subtrees: batch * d (d = 22 for SHA2-128f)
+    assert out.stdout.split() == ["4", "4", "16", "1032", "16"]
+
+
+def test_autotune_table_compaction():
+    from cuspx_amd.autotune import compact
+    rows = [("s", "sign", 22, 8), ("s", "sign", 88, 8), ("s", "sign", 352, 4),
+            ("s", "sign", 1408, 4), ("s", "sign", 5632, 1), ("s", "verify", 1, 1032),
+            ("s", "verify", 4096, 16)]
+    assert compact(rows) == [("s", "sign", 0, 8), ("s", "sign", 352, 4), ("s", "sign", 5632, 1),
+                             ("s", "verify", 0, 1032), ("s", "verify", 4096, 16)]
