@@ -37,16 +37,17 @@ PEAK_VOP3_T = SIMDS * 16 * CLK_HZ / 1e12    # 39.32 T lane-instr/s (4-cycle clas
 # units of one batch-4096 launch (profiles/k_sign_fused_sha2_128f_b4096.md,
 # profiles/k_sign_fused_shake256_128f_b4096.md)
 SHA256_OPS = 1237
-KECCAK_OPS = 4276
+KECCAK_OPS = 4163
 SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
 # Issue rate of independent instruction blocks in the proportions of the unit's
 # loop body (wave-instructions per SIMD per ns, cuspx_amd/valu_rate "mix" rows):
 # SHA-256 F = 717 four-cycle (508 alignbit, 205 add3, 4 perm) : 506 two-cycle
-# (312 bitop3, 194 VOP2) ~ 7:5; Keccak-f round = 58 alignbit : 122 bitop3/xor ~ 1:2.
+# (312 bitop3, 194 VOP2) ~ 7:5; Keccak-f round of a WOTS chain step (bit-interleaved)
+# = 52 alignbit : 122 bitop3/xor ~ 3:7.
 # Mixing the two classes issues slower than the sum of their separate rates,
 # so this measured rate, not a 4c/2c model, is the ceiling of a unit.
 MIX_RATE_SHA256 = 0.6235
-MIX_RATE_KECCAK = 0.7253
+MIX_RATE_KECCAK = 0.7501
 
 
 def mix_ceiling_units(rate, ops_per_unit):

@@ -121,6 +121,158 @@ __device__ __forceinline__ void keccak_f(uint32_t s[50]) {
     for (int r = 1; r < 24; r++) keccak_round(s, kKeccakRC[2 * r], kKeccakRC[2 * r + 1]);
 }
 
+// ---------------------------------------------------- bit-interleaved form
+//
+// A 64-bit lane (lo, hi) can also be held as (E, O): E = its even-numbered
+// bits, O = its odd-numbered bits.  A 64-bit rotation by 2k is then a 32-bit
+// rotation of both halves by k, and by 2k+1 a swap plus rotations by k+1 and
+// k -- theta's rotation by 1 costs one v_alignbit instead of two, and so does
+// rho's rotation by 1: 52 instead of 58 rotates per round.  The WOTS chain
+// steps of the hypertree kernel run in this form (values stay interleaved
+// from step to step); everything else uses the plain form.
+
+// bits 0, 2, 4, ... of x into bits 0..15
+__device__ __forceinline__ uint32_t il_even16(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0x0000FFFFu;
+}
+// bits 0..15 of x into bits 0, 2, 4, ...
+__device__ __forceinline__ uint32_t il_spread16(uint32_t x) {
+    x &= 0x0000FFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    return (x | (x << 1)) & 0x55555555u;
+}
+// (lo, hi) -> (E, O)
+__device__ __forceinline__ void il_to(uint32_t lo, uint32_t hi, uint32_t& e, uint32_t& o) {
+    e = il_even16(lo) | (il_even16(hi) << 16);
+    o = il_even16(lo >> 1) | (il_even16(hi >> 1) << 16);
+}
+// (E, O) -> (lo, hi)
+__device__ __forceinline__ void il_from(uint32_t e, uint32_t o, uint32_t& lo, uint32_t& hi) {
+    lo = il_spread16(e) | (il_spread16(o) << 1);
+    hi = il_spread16(e >> 16) | (il_spread16(o >> 16) << 1);
+}
+
+// rotl32 by M (M in 0..32)
+template <int M>
+__device__ __forceinline__ uint32_t rotl32c(uint32_t x) {
+    if constexpr (M == 0 || M == 32) return x;
+    else return __builtin_amdgcn_alignbit(x, x, 32 - M);
+}
+
+// 64-bit rotation by R of an interleaved lane
+template <int R>
+__device__ __forceinline__ void rol64_il(uint32_t e, uint32_t o, uint32_t& oe, uint32_t& oo) {
+    if constexpr ((R & 1) == 0) {
+        oe = rotl32c<R / 2>(e);
+        oo = rotl32c<R / 2>(o);
+    } else {
+        oe = rotl32c<(R + 1) / 2>(o);
+        oo = rotl32c<(R - 1) / 2>(e);
+    }
+}
+
+// round constants in interleaved form: (E, O) of each 64-bit constant
+struct KeccakRcIl {
+    uint32_t w[48];
+    constexpr KeccakRcIl() : w{} {
+        constexpr uint64_t rc[24] = {
+            0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808aull,
+            0x8000000080008000ull, 0x000000000000808bull, 0x0000000080000001ull,
+            0x8000000080008081ull, 0x8000000000008009ull, 0x000000000000008aull,
+            0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000aull,
+            0x000000008000808bull, 0x800000000000008bull, 0x8000000000008089ull,
+            0x8000000000008003ull, 0x8000000000008002ull, 0x8000000000000080ull,
+            0x000000000000800aull, 0x800000008000000aull, 0x8000000080008081ull,
+            0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
+        for (int r = 0; r < 24; r++) {
+            uint32_t e = 0, o = 0;
+            for (int b = 0; b < 32; b++) {
+                e |= (uint32_t)((rc[r] >> (2 * b)) & 1u) << b;
+                o |= (uint32_t)((rc[r] >> (2 * b + 1)) & 1u) << b;
+            }
+            w[2 * r] = e;
+            w[2 * r + 1] = o;
+        }
+    }
+};
+__constant__ KeccakRcIl kKeccakRcIl = KeccakRcIl();
+
+// one round on the interleaved state (s[2i] = E, s[2i+1] = O of lane i)
+__device__ __forceinline__ void keccak_round_il(uint32_t s[50], uint32_t rc_e, uint32_t rc_o) {
+    uint32_t C[10];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            C[2 * x + h] = xor3(xor3(s[2 * x + h], s[2 * (x + 5) + h], s[2 * (x + 10) + h]),
+                                s[2 * (x + 15) + h], s[2 * (x + 20) + h]);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        const int xp = (x + 1) % 5, xm = (x + 4) % 5;
+        const uint32_t re = rotl32c<1>(C[2 * xp + 1]), ro = C[2 * xp];  // rol64 by 1
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            s[2 * (x + 5 * y)] = xor3(s[2 * (x + 5 * y)], C[2 * xm], re);
+            s[2 * (x + 5 * y) + 1] = xor3(s[2 * (x + 5 * y) + 1], C[2 * xm + 1], ro);
+        }
+    }
+    uint32_t B[50];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            const int src = x + 5 * y;
+            const int dst = y + 5 * ((2 * x + 3 * y) % 5);
+            uint32_t e, o;
+            switch (kRho[src]) {
+#define SPX_ROT_CASE(R) \
+    case R: rol64_il<R>(s[2 * src], s[2 * src + 1], e, o); break;
+                SPX_ROT_CASE(0) SPX_ROT_CASE(1) SPX_ROT_CASE(62) SPX_ROT_CASE(28)
+                SPX_ROT_CASE(27) SPX_ROT_CASE(36) SPX_ROT_CASE(44) SPX_ROT_CASE(6)
+                SPX_ROT_CASE(55) SPX_ROT_CASE(20) SPX_ROT_CASE(3) SPX_ROT_CASE(10)
+                SPX_ROT_CASE(43) SPX_ROT_CASE(25) SPX_ROT_CASE(39) SPX_ROT_CASE(41)
+                SPX_ROT_CASE(45) SPX_ROT_CASE(15) SPX_ROT_CASE(21) SPX_ROT_CASE(8)
+                SPX_ROT_CASE(18) SPX_ROT_CASE(2) SPX_ROT_CASE(61) SPX_ROT_CASE(56)
+                SPX_ROT_CASE(14)
+#undef SPX_ROT_CASE
+                default: e = s[2 * src]; o = s[2 * src + 1]; break;
+            }
+            B[2 * dst] = e;
+            B[2 * dst + 1] = o;
+        }
+    }
+#pragma unroll
+    for (int y = 0; y < 5; y++) {
+#pragma unroll
+        for (int x = 0; x < 5; x++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                s[2 * (x + 5 * y) + h] = bit_chi(B[2 * (x + 5 * y) + h],
+                                                 B[2 * ((x + 1) % 5 + 5 * y) + h],
+                                                 B[2 * ((x + 2) % 5 + 5 * y) + h]);
+            }
+        }
+    }
+    s[0] ^= rc_e;
+    s[1] ^= rc_o;
+}
+
+// Keccak-f[1600] on the interleaved state; round 0 unrolled as in keccak_f
+__device__ __forceinline__ void keccak_f_il(uint32_t s[50]) {
+    keccak_round_il(s, 1u, 0u);  // RC[0] = 1: bit 0 is even
+#pragma unroll 1
+    for (int r = 1; r < 24; r++)
+        keccak_round_il(s, kKeccakRcIl.w[2 * r], kKeccakRcIl.w[2 * r + 1]);
+}
+
 // xor x into s[q] for a wave-uniform runtime q < 34 (the rate)
 __device__ __forceinline__ void xor_at(uint32_t s[50], int q, uint32_t x) {
 #pragma unroll
@@ -175,6 +327,55 @@ struct Shake {
     __device__ static __forceinline__ void F(const Key& k, const Adr& a, const uint32_t in[NW],
                                              uint32_t out[NW]) {
         one_block<NW>(k.pub, a, in, out);
+    }
+
+    // WOTS chain in the interleaved form: pub_seed and the chain's address
+    // (hash field 0) are interleaved once; each step adds the hash field --
+    // address byte 21, bits 40..43 of sponge lane (n + 21) / 8 -- by xor,
+    // which commutes with interleaving.  Values in and out are interleaved.
+    static constexpr int kHashLane = (N + 21) / 8;
+    static constexpr int kPadLane = (2 * N + 32) / 8;  // the 0x1F byte, at a lane start
+    struct ChainIl {
+        uint32_t w[NW + 8];  // (E, O) of the pub_seed and address lanes
+    };
+    __device__ static __forceinline__ void chain_il_begin(const Key& k, const Adr& a,
+                                                          ChainIl& c) {
+#pragma unroll
+        for (int l = 0; l < (NW + 8) / 2; l++) {
+            const uint32_t lo = 2 * l < NW ? k.pub[2 * l] : aw(a, 2 * l - NW);
+            const uint32_t hi = 2 * l + 1 < NW ? k.pub[2 * l + 1] : aw(a, 2 * l + 1 - NW);
+            il_to(lo, hi, c.w[2 * l], c.w[2 * l + 1]);
+        }
+    }
+    // one chain step at hash address h (h < 16), in and out interleaved
+    __device__ static __forceinline__ void F_il(const ChainIl& c, uint32_t h,
+                                                const uint32_t in[NW], uint32_t out[NW]) {
+        uint32_t s[50];
+#pragma unroll
+        for (int i = 0; i < 50; i++) {
+            uint32_t x = 0;
+            if (i < NW + 8) x = c.w[i];
+            else if (i < 2 * NW + 8) x = in[i - NW - 8];
+            if (i == 2 * kPadLane) x ^= 7u;      // 0x1F: even bits 0, 2, 4
+            if (i == 2 * kPadLane + 1) x ^= 3u;  //       odd bits 1, 3
+            if (i == 33) x ^= 0x80000000u;       // 0x80 at byte 135: lane 16, bit 63
+            s[i] = x;
+        }
+        s[2 * kHashLane] ^= ((h & 1u) << 20) | (((h >> 2) & 1u) << 21);
+        s[2 * kHashLane + 1] ^= (((h >> 1) & 1u) << 20) | (((h >> 3) & 1u) << 21);
+        keccak_f_il(s);
+#pragma unroll
+        for (int i = 0; i < NW; i++) out[i] = s[i];
+    }
+    // NW plain words (NW / 2 lanes) <-> interleaved
+    __device__ static __forceinline__ void to_il(const uint32_t in[NW], uint32_t out[NW]) {
+#pragma unroll
+        for (int l = 0; l < NW / 2; l++) il_to(in[2 * l], in[2 * l + 1], out[2 * l], out[2 * l + 1]);
+    }
+    __device__ static __forceinline__ void from_il(const uint32_t in[NW], uint32_t out[NW]) {
+#pragma unroll
+        for (int l = 0; l < NW / 2; l++)
+            il_from(in[2 * l], in[2 * l + 1], out[2 * l], out[2 * l + 1]);
     }
 
     __device__ static __forceinline__ void H(const Key& k, const Adr& a, const uint32_t l[NW],

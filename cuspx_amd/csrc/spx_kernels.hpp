@@ -563,12 +563,28 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
         adr_chain(wa, j);
         adr_hash(wa, 0);
         Hh::PRF(key, wa, v);  // wots_gen_sk, wots.cu:27-33
-        if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16) * P::n, v);
+        if constexpr (P::hash == kSha2) {
+            if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16) * P::n, v);
 #pragma unroll 1
-        for (int s = 0; s < P::w - 1; s++) {  // gen_chain(0, w-1), wots.cu:51-62
-            adr_hash(wa, s);
-            Hh::F(key, wa, v, v);
-            if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16 + s + 1) * P::n, v);
+            for (int s = 0; s < P::w - 1; s++) {  // gen_chain(0, w-1), wots.cu:51-62
+                adr_hash(wa, s);
+                Hh::F(key, wa, v, v);
+                if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16 + s + 1) * P::n, v);
+            }
+        } else {
+            // SHAKE: the chain runs bit-interleaved; the capture keeps that
+            // form (k_wots_gather converts the values it picks)
+            typename Hh::ChainIl ch;
+            Hh::chain_il_begin(key, wa, ch);
+            uint32_t vi[NW];
+            Hh::to_il(v, vi);
+            if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16) * P::n, vi);
+#pragma unroll 1
+            for (int s = 0; s < P::w - 1; s++) {  // gen_chain(0, w-1), wots.cu:51-62
+                Hh::F_il(ch, s, vi, vi);
+                if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16 + s + 1) * P::n, vi);
+            }
+            Hh::from_il(vi, v);
         }
         if constexpr (!Sh::park) {
 #pragma unroll 1
@@ -707,8 +723,17 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     uint32_t* dst = reinterpret_cast<uint32_t*>(A.sig + (size_t)req * A.sig_stride + P::n +
                                                 P::fors_bytes + (size_t)layer * P::layer_bytes +
                                                 (size_t)j * P::n);
+    if constexpr (P::hash == kShake) {  // SHAKE chains are captured bit-interleaved
+        uint32_t vi[NW], v[NW];
 #pragma unroll
-    for (int w = 0; w < NW; w++) dst[w] = src[w];
+        for (int w = 0; w < NW; w++) vi[w] = src[w];
+        Shake<P::n>::from_il(vi, v);
+#pragma unroll
+        for (int w = 0; w < NW; w++) dst[w] = v[w];
+    } else {
+#pragma unroll
+        for (int w = 0; w < NW; w++) dst[w] = src[w];
+    }
 }
 
 // verify: G lanes per request walk the d layers in order (sign.cu:3077-3113).

@@ -83,6 +83,9 @@ KERNEL(k_mix_sha, S_("v41") F_("v42") S_("v43") A_("v45") F_("v46") S_("v47") V_
 // 1 four-cycle : 2 two-cycle, the proportion of a Keccak-f round (58 : 122)
 KERNEL(k_mix_keccak, S_("v41") F_("v42") F_("v43") S_("v45") F_("v46") F_("v47") S_("v49")
                          F_("v50") F_("v51") S_("v53") F_("v54") F_("v55"))
+// 3 four-cycle : 7 two-cycle, a bit-interleaved Keccak-f round (52 : 122)
+KERNEL(k_mix_keccak_il, S_("v41") F_("v42") F_("v43") S_("v45") F_("v46") F_("v47")
+                            S_("v49") F_("v50") F_("v51") F_("v53"))
 
 #define CHECK(x)                                                                     \
     do {                                                                             \
@@ -124,6 +127,7 @@ int main() {
         {"mix S F S F ..", k_mix_sf, 8},      {"mix S S F F ..", k_mix_ssff, 8},
         {"mix S F F F ..", k_mix_sfff, 8},    {"mix 7 S : 5 F (SHA-256)", k_mix_sha, 12},
         {"mix 1 S : 2 F (Keccak)", k_mix_keccak, 12},
+        {"mix 3 S : 7 F (Keccak, interleaved)", k_mix_keccak_il, 10},
     };
     printf("device: %d CUs, %.0f MHz; S = 4-cycle form, F = 2-cycle form\n", cus,
            clk_khz / 1000.0);
