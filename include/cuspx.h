@@ -122,7 +122,10 @@ int spx_batch_verify(int pid, uint32_t num, const uint8_t *pk, const uint8_t *si
  * hipStream_t (NULL = the library's stream).  msg + moff[i] must be 4-byte
  * aligned and readable up to the next 4-byte boundary.  ws is a workspace
  * of cuspx_workspace_bytes(pid, num, op) bytes (op 0 sign, 1 verify,
- * 2 keygen).  Calls are asynchronous on the stream. */
+ * 2 keygen); it holds the batch's plans, intermediate nodes and work
+ * counters, so launches that may overlap need workspaces of their own (a
+ * graph keeps its workspace for every replay).  Calls are asynchronous on
+ * the stream. */
 size_t cuspx_workspace_bytes(int pid, uint32_t num, int op);
 int cuspx_sign_device(int pid, uint32_t num, const uint8_t *sk, uint32_t sk_stride,
                       const uint8_t *msg, const uint64_t *moff, const uint32_t *mlen,
