@@ -133,7 +133,7 @@ def make_inputs(C, pid, batch):
 SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
 
 
-def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_us=500,
+def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_us=1000,
                 seed=5):
     """Serving workload: 256 resident keys over SHA2-128f / SHA2-128s /
     SHAKE256-256f, Poisson arrivals at `qps`, 80% sign / 20% verify, message
@@ -342,7 +342,7 @@ def main():
     ap.add_argument("--mixed", type=int, default=8192)
     ap.add_argument("--graph-128s", type=int, default=1)
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-window-us", type=int, default=500)
+    ap.add_argument("--serve-window-us", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
