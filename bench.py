@@ -362,7 +362,7 @@ def main():
     ap.add_argument("--graph-128s", type=int, default=1)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-window-us", type=int, default=1000)
-    ap.add_argument("--serve-sweep", default="4000,6000,8000",
+    ap.add_argument("--serve-sweep", default="4000,6000,8000,10000,12000",
                     help="offered rates (req/s) tried beyond --serve-qps, 3 s each")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -144,9 +144,9 @@ def set_default_param(pid):
 
 def set_split(pid, op, lanes_per_leaf):
     """Force a work split (0 = automatic / tuned): lanes per WOTS leaf of the
-    hypertree kernel for OP_SIGN / OP_KEYGEN (1, 2, 4, 8, 16); lanes per request of
-    the verify kernel for OP_VERIFY (16..128, or 1000 + lanes for the
-    one-hash-unit-loop kernel)."""
+    hypertree kernel for OP_SIGN / OP_KEYGEN (1, 2, 4, 8, 16); mean lanes per
+    request of the verify kernel for OP_VERIFY (16, 32, 64; 128 when len = 67),
+    which runs 256 / lanes requests per workgroup."""
     rc = lib().cuspx_set_split(pid, op, lanes_per_leaf)
     if rc != OK:
         raise CuspxError("cuspx_set_split failed (%d)" % rc)

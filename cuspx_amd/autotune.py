@@ -3,9 +3,9 @@
 k_ht gives each WOTS leaf S lanes (S in 1, 2, 4, 8, 16; each lane walks
 ceil(len/S) chains). A small batch fills the chip only with a wider split; a
 large one runs fastest at S=1, which needs no cross-lane reduction. k_vht
-gives each request G lanes (16 to 128): wider groups shorten each layer's
-chain phase, narrower ones pack more requests per wave; 1000 + G selects
-k_vht_loop, which overlaps T_len with the chains. The crossovers depend
+runs 256 / G requests per workgroup (G = 16 to 128 lanes per request on
+average): wider groups shorten each layer's chain phase down to the longest
+chain, narrower ones keep more requests busy per lane. The crossovers depend
 on the set and on the batch, so they are measured: for every set, operation
 and batch size this times device-resident launches at each split the set
 allows and writes the winners as
@@ -66,11 +66,9 @@ def tune_set(name, batches, reps, log):
     C.sign_device(pid, B, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr, d_opt.ptr,
                   d_sig.ptr, p.sig_bytes, ws.ptr, sh)
     stream.sync()
-    lo = 32 if p.len > 40 else 16
     hi = 128 if p.len > 64 else 64
-    vw = [G for G in (16, 32, 64, 128) if lo <= G <= hi]
-    vw += [1000 + G for G in (16, 32, 64) if G >= lo]  # the one-hash-unit-loop kernel
-    widths = {0: (1, 2, 4, 8, 16), 2: (1, 2, 4, 8, 16), 1: tuple(vw)}
+    widths = {0: (1, 2, 4, 8, 16), 2: (1, 2, 4, 8, 16),
+              1: tuple(G for G in (16, 32, 64, 128) if G <= hi)}
     rows = []
     for op_name, op in OPS.items():
         for b in batches:

@@ -8,9 +8,6 @@
 namespace spx {
 
 enum { kOpSign = 0, kOpVerify = 1, kOpKeygen = 2 };
-// verify split values at or above kVhtLoop select the one-hash-unit-loop
-// verify kernel with (value - kVhtLoop) lanes per request
-enum { kVhtLoop = 1000 };
 
 // status codes of every C-ABI entry point
 enum {

@@ -4,7 +4,6 @@
 #include "engine.hpp"
 #include "profile.hpp"
 #include "spx_kernels.hpp"
-#include "vht_loop.hpp"
 
 namespace spx {
 
@@ -32,21 +31,15 @@ struct Layout {
         while (S < max_split() && leaves * S * 2 <= kFillLanes) S *= 2;
         return S;
     }
-    // Verify kernel and lanes per request.  G in [vmin, vmax] selects the
-    // layer-synchronous k_vht with G lanes; kVhtLoop + G (G in [vmin, 64])
-    // selects k_vht_loop, the one-hash-unit loop that overlaps T_len with the
-    // chains.  A tuned value when the table has one,
-    // else k_vht with the base width (more lanes for the long chain lists of
-    // n = 24, 32) doubled while the launch would fill under an eighth of the
-    // chip.  Wider groups shorten each layer's chain phase at the cost of
-    // idle lanes.
-    static constexpr int vmin() { return P::len > 40 ? 32 : 16; }
+    // Verify kernel width: G lanes per request on average (k_vht runs
+    // 256 / G requests per workgroup and deals all their chains over its 256
+    // lanes).  A tuned value when the table has one, else G = 16 doubled
+    // while the launch would fill under an eighth of the chip: wider groups
+    // shorten each layer's chain phase down to the longest chain, narrower
+    // ones keep more requests per lane.
+    static constexpr int vmin() { return 16; }
     static constexpr int vmax() { return P::len > 64 ? 128 : 64; }
-    static bool vvalid(int t) {
-        const int g = t >= kVhtLoop ? t - kVhtLoop : t;
-        if (g < vmin() || (g & (g - 1)) != 0) return false;
-        return t >= kVhtLoop ? g <= 64 : g <= vmax();
-    }
+    static bool vvalid(int g) { return g >= vmin() && g <= vmax() && (g & (g - 1)) == 0; }
     static int vsplit(int pid, uint32_t n) {
         const int t = tuned_split(pid, kOpVerify, n);
         if (vvalid(t)) return t;
@@ -203,12 +196,6 @@ struct SetImpl {
     static void launch_vht(const BatchArgs& A, uint32_t n, hipStream_t s) {
         hipLaunchKernelGGL((k_vht<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
     }
-    template <int G>
-    static void launch_vloop(const BatchArgs& A, uint32_t n, hipStream_t s) {
-        if constexpr (G <= 64)
-            hipLaunchKernelGGL((k_vht_loop<P, G>), dim3(cdiv(n, 256 / G)), dim3(256), 0, s, A);
-    }
-
     static int sign(const DevBatch& b, hipStream_t s) {
         BatchArgs A{};
         A.sk = b.sk;
@@ -275,19 +262,10 @@ struct SetImpl {
         prof_mark(s, "k_vfors");
         hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
-        constexpr int V0 = Lay::vmin();
-        if (G >= kVhtLoop) {
-            const int g = G - kVhtLoop;
-            if (g == V0) launch_vloop<V0>(A, n, s);
-            else if (g == 2 * V0) launch_vloop<2 * V0>(A, n, s);
-            else launch_vloop<4 * V0>(A, n, s);
-        } else if (G == V0) {
-            launch_vht<V0>(A, n, s);
-        } else if (G == 2 * V0 || 4 * V0 > Lay::vmax()) {
-            launch_vht<2 * V0>(A, n, s);
-        } else if constexpr (4 * V0 <= Lay::vmax()) {
-            launch_vht<4 * V0>(A, n, s);
-        }
+        if (G == 16) launch_vht<16>(A, n, s);
+        else if (G == 32) launch_vht<32>(A, n, s);
+        else if (G == 64) launch_vht<64>(A, n, s);
+        else if constexpr (Lay::vmax() >= 128) launch_vht<128>(A, n, s);
         prof_mark(s, "k_vht");
         return hipGetLastError() == hipSuccess ? 0 : kErrHip;
     }

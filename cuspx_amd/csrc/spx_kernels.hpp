@@ -12,8 +12,8 @@
 //                        the signing leaf's chain values are captured
 //           k_wots_gather lane per (request, layer, chain): picks the
 //                        captured chain value for the layer's message
-//   verify: k_vprep, k_vfors, k_fors_pk, k_vht (len lanes per request, layers
-//           in sequence)
+//   verify: k_vprep, k_vfors, k_fors_pk, k_vht (the chains of a workgroup's
+//           requests dealt over its lanes, layers in sequence)
 //   keygen: k_kprep, k_ht in keygen mode (top layer only)
 //
 // Semantics follow sign.cu:1924-1994 (sign), sign.cu:3048-3121 (verify),
@@ -736,160 +736,334 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     }
 }
 
-// verify: G lanes per request walk the d layers in order (sign.cu:3077-3113).
-// Per layer the len WOTS chains (15 - len_j steps each, data dependent) are
-// ranked by remaining steps in LDS and dealt to the G lanes in snake order,
-// a longest-first schedule that evens the per-lane F counts; each lane runs
-// its chains in one flattened loop (one F site).  One lane per request then
-// computes T_len and climbs the subtree (compute_root, utils.cu:94-135).
+// T_len of one WOTS public key held in LDS as native words (thash with the
+// WOTS_PK address, thash_sha256.cu:16-34): one compression per iteration;
+// blocks of payload words only cost one v_alignbit per word.
+template <class P>
+__device__ __forceinline__ void tlen_lds(const uint32_t mid[8], const Adr& a, const uint32_t* pk,
+                                         uint32_t out[P::nw]) {
+    constexpr int M = P::len * P::nw;
+    constexpr int B = (6 + M + 2 + 15) / 16;             // compression blocks
+    constexpr int FT = (5 + M - 15 + 15) / 16;           // first block that is not payload only
+    constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+    static_assert(FT >= 1 && FT < B, "T_len spans a head, payload and tail blocks");
+    uint32_t st[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) st[i] = mid[i];
+#pragma unroll 1
+    for (int blk = 0; blk < B; blk++) {
+        uint32_t w[16];
+        if (blk == 0) {
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                w[i] = i < 5 ? a.w[i < 5 ? i : 0]
+                     : i == 5 ? (a.w[5] | (pk[0] >> 16))
+                              : __builtin_amdgcn_alignbit(pk[i - 6 > 0 ? i - 6 : 0], pk[i - 5], 16);
+        } else if (blk < FT) {
+            const uint32_t* p = pk + 16 * blk - 6;
+#pragma unroll
+            for (int i = 0; i < 16; i++) w[i] = __builtin_amdgcn_alignbit(p[i], p[i + 1], 16);
+        } else {
+#pragma unroll
+            for (int tb = FT; tb < B; tb++) {
+                if (blk == tb) {
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const int g = 16 * tb + i;
+                        uint32_t x = 0;
+                        if (g < 5 + M)
+                            x = __builtin_amdgcn_alignbit(pk[g - 6], pk[g - 5 < M ? g - 5 : M - 1], 16);
+                        else if (g == 5 + M) x = (pk[M - 1] << 16) | 0x8000u;
+                        else if (g == 16 * B - 1) x = bits;
+                        w[i] = x;
+                    }
+                }
+            }
+        }
+        sha256_compress(st, w);
+    }
+#pragma unroll
+    for (int i = 0; i < P::nw; i++) out[i] = st[i];
+}
+
+// SHAKE256 T_len of one WOTS public key held in LDS as bit-interleaved words
+// (thash_shake256.cu:15-25: pub_seed || addr32 || pk), absorbed 136 bytes per
+// permutation in the interleaved domain; the output is a plain value.
+template <class P>
+__device__ __forceinline__ void tlen_lds_il(const typename Shake<P::n>::Key& key, const Adr& a,
+                                            const uint32_t* pk, uint32_t out[P::nw]) {
+    using Hh = Shake<P::n>;
+    constexpr int NW = P::nw;
+    constexpr int M = P::len * NW;
+    constexpr int T = NW + 8 + M;             // absorbed words
+    constexpr int NB = T / 34 + 1;            // permutations (the pad always fits)
+    constexpr int TAIL = T - 34 * (NB - 1);   // words in the last block: the 0x1F byte follows
+    static_assert(NB >= 2, "T_len spans at least two blocks");
+    typename Hh::ChainIl pre;                 // pub_seed || address, interleaved
+    Hh::chain_il_begin(key, a, pre);
+    uint32_t s[50];
+#pragma unroll
+    for (int i = 0; i < 50; i++) s[i] = 0;
+#pragma unroll 1
+    for (int blk = 0; blk < NB; blk++) {
+        if (blk == 0) {
+#pragma unroll
+            for (int i = 0; i < 34; i++) s[i] ^= i < NW + 8 ? pre.w[i < NW + 8 ? i : 0] : pk[i - NW - 8];
+        } else if (blk < NB - 1) {
+            const uint32_t* p = pk + 34 * blk - NW - 8;
+#pragma unroll
+            for (int i = 0; i < 34; i++) s[i] ^= p[i];
+        } else {
+            constexpr int base = 34 * (NB - 1) - NW - 8;
+#pragma unroll
+            for (int i = 0; i < TAIL; i++) s[i] ^= pk[base + i];
+            if constexpr ((TAIL & 1) == 0) {  // 0x1F at bit 0 of lane TAIL / 2
+                s[TAIL] ^= 7u;
+                s[TAIL + 1] ^= 3u;
+            } else {                          // 0x1F at bit 32 of lane TAIL / 2
+                s[TAIL - 1] ^= 7u << 16;
+                s[TAIL] ^= 3u << 16;
+            }
+            s[33] ^= 0x80000000u;             // 0x80 at byte 135: lane 16, bit 63
+        }
+        keccak_f_il(s);
+    }
+    Hh::from_il(s, out);
+}
+
+// verify: the d layers of R = 256 / G requests per workgroup in order
+// (sign.cu:3077-3113), so G is the mean number of lanes per request.  Per
+// layer the R * len WOTS chains of the workgroup (15 - len_j steps each,
+// data dependent) are ranked by remaining steps with a counting sort in LDS
+// and dealt to the 256 lanes in snake order: a longest-first schedule that
+// evens the per-lane F counts across all requests of the workgroup, not per
+// request.  Each lane runs its chains in one flattened loop (one F site;
+// SHAKE256 chains run bit-interleaved).  Lane r < R then computes T_len of
+// request r block by block from LDS and climbs its subtree (compute_root,
+// utils.cu:94-135).  The next layer's WOTS signature words and this layer's
+// auth path are fetched into registers while the chains run, so the layer
+// loop waits on no global load.
 template <class P, int G>
 __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
+    constexpr bool SHA2 = P::hash == kSha2;
     constexpr int NW = P::nw;
     constexpr int LEN = P::len;
-    constexpr int R = 256 / G;                 // requests per workgroup
-    constexpr int MAXC = (LEN + G - 1) / G;    // chains per lane
+    constexpr int TH = P::th;
+    constexpr int R = 256 / G;                   // requests per workgroup
+    constexpr int TLW = LEN * NW;                // WOTS signature / public key words
+    constexpr int NCH = R * LEN;                 // chains per workgroup and layer
+    constexpr int MAXC = (NCH + 255) / 256;      // chains per lane
+    constexpr int MAXW = (R * TLW + 255) / 256;  // prefetched WOTS words per lane
+    constexpr int AW = TH * NW;                  // auth path words per request
+    constexpr int MAXA = (R * AW + 255) / 256;   // prefetched auth words per lane
+    static_assert(NCH < 4096, "chain ids take 12 bits of an order entry");
     __builtin_amdgcn_s_setprio(2);  // d dependent layers per request: latency-bound
-    __shared__ uint32_t pkbuf[R][LEN * NW];
-    __shared__ uint32_t rootb[R][NW];
-    __shared__ uint8_t steps[R][LEN];
-    __shared__ uint8_t order[R][LEN];
+    __shared__ uint32_t key_s[R][8];    // SHA-256 midstate / SHAKE256 pub_seed
+    __shared__ uint32_t root_s[R][NW];  // root of the last finished layer
+    __shared__ uint64_t tree_s[R];
+    __shared__ uint32_t leaf_s[R];
+    __shared__ uint32_t sig_s[R][TLW];  // WOTS signature of the layer, raw words
+    __shared__ uint32_t pk_s[R][TLW];   // chain ends (SHAKE256: interleaved)
+    __shared__ uint32_t auth_s[R][AW];  // auth path of the layer, raw words
+    __shared__ uint16_t order_s[NCH];   // rank -> chain id | remaining steps << 12
+    __shared__ uint32_t hist_s[16];     // chains per remaining-step count
 
     const int tid = threadIdx.x;
-    const int grp = tid / G;
-    const int l = tid % G;
-    const uint32_t req = blockIdx.x * R + grp;
-    const bool active = req < A.n_req;
-    const uint32_t rq = active ? req : 0;
-    const Plan& pl = A.plan[rq];
-    Key key;
-    Hh::load_key(pl, key);
-    const uint8_t* sig = A.sig + (size_t)rq * A.sig_stride;
-    if (l == 0) {
-        uint32_t fp[NW];
-        load_val<P::hash, NW>(A.fors_pk + (size_t)rq * P::n, fp);
+    const uint32_t req0 = blockIdx.x * R;
+    const int nact = (int)min((uint32_t)R, A.n_req - req0);  // >= 1 by the grid
+    const int nch = nact * LEN;
+    auto sig_of = [&](int r) { return A.sig + (size_t)(req0 + r) * A.sig_stride; };
+    auto wots_of = [&](int r, int layer) {
+        return reinterpret_cast<const uint32_t*>(sig_of(r) + P::n + P::fors_bytes +
+                                                 (size_t)layer * P::layer_bytes);
+    };
+
+    if (tid < nact) {
+        const Plan& pl = A.plan[req0 + tid];
 #pragma unroll
-        for (int w = 0; w < NW; w++) rootb[grp][w] = fp[w];
+        for (int i = 0; i < 8; i++) key_s[tid][i] = SHA2 ? pl.mid[i] : pl.pubseed[i];
+        uint32_t fp[NW];
+        load_val<P::hash, NW>(A.fors_pk + (size_t)(req0 + tid) * P::n, fp);
+#pragma unroll
+        for (int w = 0; w < NW; w++) root_s[tid][w] = fp[w];
+        tree_s[tid] = pl.tree;
+        leaf_s[tid] = pl.leaf;
     }
-    uint64_t tree = pl.tree;
-    uint32_t leaf = pl.leaf;
+    if (tid < 16) hist_s[tid] = 0;
+#pragma unroll
+    for (int k = 0; k < MAXW; k++) {
+        const int u = tid + 256 * k;
+        if (u < nact * TLW) sig_s[u / TLW][u % TLW] = wots_of(u / TLW, 0)[u % TLW];
+    }
+
 #pragma unroll 1
     for (int layer = 0; layer < P::d; layer++) {
-        __syncthreads();
-        const uint8_t* lsig = sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes;
-        uint32_t msg[NW];
+        __syncthreads();  // roots, tree / leaf, signature words and histogram ready
+        // remaining steps of every chain and its slot within its step count
+        uint32_t cst[MAXC], cslot[MAXC];
 #pragma unroll
-        for (int w = 0; w < NW; w++) msg[w] = rootb[grp][w];
-        // remaining steps per chain
+        for (int k = 0; k < MAXC; k++) {
+            const int q = tid + 256 * k;
+            cst[k] = 0;
+            cslot[k] = 0;
+            if (q < nch) {
+                uint32_t msg[NW];
 #pragma unroll
-        for (int c = 0; c < MAXC; c++) {
-            const int j = l + c * G;
-            if (j < LEN) steps[grp][j] = (uint8_t)(P::w - 1 - chain_len<P>(msg, j));
-        }
-        __syncthreads();
-        // rank: more steps first, ties by chain index
-#pragma unroll
-        for (int c = 0; c < MAXC; c++) {
-            const int j = l + c * G;
-            if (j < LEN) {
-                const uint32_t sj = steps[grp][j];
-                int rank = 0;
-#pragma unroll 5
-                for (int i = 0; i < LEN; i++) {
-                    const uint32_t si = steps[grp][i];
-                    rank += (si > sj) || (si == sj && i < j);
-                }
-                order[grp][rank] = (uint8_t)j;
+                for (int w = 0; w < NW; w++) msg[w] = root_s[q / LEN][w];
+                cst[k] = P::w - 1 - chain_len<P>(msg, q % LEN);
+                cslot[k] = atomicAdd(&hist_s[15 - cst[k]], 1u);
             }
         }
         __syncthreads();
-        // my chains: ranks l, 2G-1-l, 2G+l, 4G-1-l, ... (snake order)
-        int mine[MAXC];
-        uint32_t pre[MAXC][NW];
+        // rank = chains with more steps + slot (longest first)
+        {
+            uint32_t h[16];
+#pragma unroll
+            for (int b = 0; b < 16; b++) h[b] = hist_s[b];
+#pragma unroll
+            for (int k = 0; k < MAXC; k++) {
+                const int q = tid + 256 * k;
+                const uint32_t bk = 15 - cst[k];
+                uint32_t start = 0;
+#pragma unroll
+                for (int b = 0; b < 15; b++) start += (uint32_t)b < bk ? h[b] : 0u;
+                if (q < nch) order_s[start + cslot[k]] = (uint16_t)(q | (cst[k] << 12));
+            }
+        }
+        __syncthreads();
+        // my chains: ranks tid, 511 - tid, 512 + tid, ... (snake order; the
+        // ranks grow with c, so the valid ones are a prefix)
         int nm = 0;
 #pragma unroll
-        for (int c = 0; c < MAXC; c++) {
-            const int r = (c & 1) ? (c + 1) * G - 1 - l : c * G + l;
-            mine[c] = r < LEN ? order[grp][r] : 0;
-            nm += r < LEN;
-            load_val<P::hash, NW>(lsig + (size_t)mine[c] * P::n, pre[c]);
+        for (int c = 0; c < MAXC; c++) nm += ((c & 1) ? (c + 1) * 256 - 1 - tid : c * 256 + tid) < nch;
+        // fetch the next layer's WOTS signature and this layer's auth path
+        const bool more = layer + 1 < P::d;
+        uint32_t nx[MAXW], au[MAXA];
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) {
+            const int u = tid + 256 * k;
+            nx[k] = (more && u < nact * TLW) ? wots_of(u / TLW, layer + 1)[u % TLW] : 0u;
         }
-        Adr wa = adr_make(layer, tree, kWots);
-        adr_keypair<P::th>(wa, leaf);
-        int c = 0;
-        int j = mine[0];
-        uint32_t s = P::w - 1 - steps[grp][j];
+#pragma unroll
+        for (int k = 0; k < MAXA; k++) {
+            const int u = tid + 256 * k;
+            au[k] = u < nact * AW ? wots_of(u / AW, layer)[TLW + u % AW] : 0u;
+        }
+        // walk the chains: one flattened loop, one F site
+        int c = 0, r = 0, j = 0;
+        uint32_t s = P::w - 1;
+        Key key;
+        Adr wa;
         uint32_t v[NW];
+        typename std::conditional<SHA2, int, typename Shake<P::n>::ChainIl>::type ch{};
 #pragma unroll
-        for (int w = 0; w < NW; w++) v[w] = pre[0][w];
+        for (int w = 0; w < NW; w++) v[w] = 0;
+#pragma unroll
+        for (int i = 0; i < NW; i++) key.sk[i] = 0;
+        auto take = [&](int cc) {  // start chain cc of this lane (everything from LDS)
+            const int rank = (cc & 1) ? (cc + 1) * 256 - 1 - tid : cc * 256 + tid;
+            const uint32_t e = order_s[rank];
+            const int q = e & 0xFFF;
+            r = q / LEN;
+            j = q % LEN;
+            s = P::w - 1 - (e >> 12);
+#pragma unroll
+            for (int w = 0; w < NW; w++) v[w] = to_native<P::hash>(sig_s[r][j * NW + w]);
+            wa = adr_make(layer, tree_s[r], kWots);
+            adr_keypair<TH>(wa, leaf_s[r]);
+            adr_chain(wa, j);
+            if constexpr (SHA2) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) key.mid[i] = key_s[r][i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < NW; i++) key.pub[i] = key_s[r][i];
+                Hh::chain_il_begin(key, wa, ch);
+                uint32_t t[NW];
+                Hh::to_il(v, t);
+#pragma unroll
+                for (int w = 0; w < NW; w++) v[w] = t[w];
+            }
+        };
+        if (nm > 0) take(0);
 #pragma unroll 1
-        while (active && c < nm) {
-            if (s >= (uint32_t)P::w - 1) {  // chain done: park it, take the next
+        while (c < nm) {
+            if (s >= (uint32_t)P::w - 1) {  // chain done: park its end, take the next
 #pragma unroll
-                for (int w = 0; w < NW; w++) pkbuf[grp][j * NW + w] = v[w];
+                for (int w = 0; w < NW; w++) pk_s[r][j * NW + w] = v[w];
                 c++;
-                if (c < nm) {
-#pragma unroll
-                    for (int cc = 1; cc < MAXC; cc++)
-                        if (c == cc) {
-                            j = mine[cc];
-#pragma unroll
-                            for (int w = 0; w < NW; w++) v[w] = pre[cc][w];
-                        }
-                    s = P::w - 1 - steps[grp][j];
-                }
+                if (c < nm) take(c);
                 continue;
             }
-            adr_chain(wa, j);
-            adr_hash(wa, s);
-            Hh::F(key, wa, v, v);
+            if constexpr (SHA2) {
+                adr_hash(wa, s);
+                Hh::F(key, wa, v, v);
+            } else {
+                Hh::F_il(ch, s, v, v);
+            }
             s++;
         }
-        __syncthreads();
-        if (active && l == 0) {
-            Adr pa = adr_make(layer, tree, kWotsPk);
-            adr_keypair<P::th>(pa, leaf);
-            uint32_t node[NW];
-            typename Hh::Stream ts;
-            if constexpr (P::hash == kSha2) {
-                ts.begin(key, pa);
-#pragma unroll 1
-                for (int q = 0; q < LEN * NW; q++) ts.word(pkbuf[grp][q]);
-                ts.template finish<LEN * NW>(node);
+#pragma unroll
+        for (int k = 0; k < MAXA; k++) {
+            const int u = tid + 256 * k;
+            if (u < nact * AW) auth_s[u / AW][u % AW] = au[k];
+        }
+        __syncthreads();  // every chain end parked; this layer's sig_s reads done
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) {
+            const int u = tid + 256 * k;
+            if (more && u < nact * TLW) sig_s[u / TLW][u % TLW] = nx[k];
+        }
+        if (tid < 16) hist_s[tid] = 0;
+        if (tid < nact) {  // T_len and the subtree climb of request tid
+            const uint64_t tree = tree_s[tid];
+            const uint32_t leaf = leaf_s[tid];
+            Key k2;
+#pragma unroll
+            for (int i = 0; i < NW; i++) k2.sk[i] = 0;
+            if constexpr (SHA2) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) k2.mid[i] = key_s[tid][i];
             } else {
-                ts.begin(key, pa, &pkbuf[grp][0], 1);
-                ts.template finish<LEN>(node);
+#pragma unroll
+                for (int i = 0; i < NW; i++) k2.pub[i] = key_s[tid][i];
             }
-            const uint8_t* auth = lsig + P::wots_bytes;
+            Adr pa = adr_make(layer, tree, kWotsPk);
+            adr_keypair<TH>(pa, leaf);
+            uint32_t node[NW];
+            if constexpr (SHA2) tlen_lds<P>(k2.mid, pa, &pk_s[tid][0], node);
+            else tlen_lds_il<P>(k2, pa, &pk_s[tid][0], node);
             uint32_t li = leaf;
             Adr ha = adr_make(layer, tree, kHashTree);
 #pragma unroll 1
-            for (int h = 0; h < P::th; h++) {
-                uint32_t ap[NW], lft[NW], rgt[NW];
-                load_val<P::hash, NW>(auth + (size_t)h * P::n, ap);
+            for (int h = 0; h < TH; h++) {
+                uint32_t lft[NW], rgt[NW];
                 const bool odd = li & 1u;
                 li >>= 1;
                 adr_node(ha, h + 1, li);
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
-                    lft[w] = odd ? ap[w] : node[w];
-                    rgt[w] = odd ? node[w] : ap[w];
+                    const uint32_t ap = to_native<P::hash>(auth_s[tid][h * NW + w]);
+                    lft[w] = odd ? ap : node[w];
+                    rgt[w] = odd ? node[w] : ap;
                 }
-                Hh::H(key, ha, lft, rgt, node);
+                Hh::H(k2, ha, lft, rgt, node);
             }
 #pragma unroll
-            for (int w = 0; w < NW; w++) rootb[grp][w] = node[w];
+            for (int w = 0; w < NW; w++) root_s[tid][w] = node[w];
+            leaf_s[tid] = (uint32_t)(tree & ((1u << TH) - 1));
+            tree_s[tid] = tree >> TH;
         }
-        leaf = (uint32_t)(tree & ((1u << P::th) - 1));
-        tree >>= P::th;
     }
     __syncthreads();
-    if (active && l == 0) {
+    if (tid < nact) {
+        const Plan& pl = A.plan[req0 + tid];
         bool ok = true;
 #pragma unroll
-        for (int w = 0; w < NW; w++) ok = ok && (rootb[grp][w] == pl.root[w]);
-        A.status[rq] = ok ? 0 : -1;
+        for (int w = 0; w < NW; w++) ok = ok && (root_s[tid][w] == pl.root[w]);
+        A.status[req0 + tid] = ok ? 0 : -1;
     }
 }
 

@@ -41,12 +41,11 @@ int cuspx_param_id(const char *name);            /* "sha2-128f", ... ; -2 if unk
 int cuspx_get_param_info(int pid, cuspx_param_info *out);
 int cuspx_set_default_param(int pid);
 /* Work split of set pid at every batch size.  op 0 (sign) / 2 (keygen): lanes
- * per WOTS leaf of the hypertree kernel (1, 2, 4, 8, 16).  op 1 (verify): lanes
- * per request of the layer-synchronous verify kernel (16, 32, 64 when
- * len <= 40; 32, 64 when len <= 64; 32, 64, 128 when len = 67); 1000 + lanes
- * (lanes 16 or 32 up to 64, as above) selects the verify kernel that runs one
- * hash unit per lane per step and overlaps T_len with the chains.  0 restores the tuning table /
- * built-in choice.  cuspx_get_split reports the choice for a batch size. */
+ * per WOTS leaf of the hypertree kernel (1, 2, 4, 8, 16).  op 1 (verify): mean
+ * lanes per request of the verify kernel, which runs 256 / lanes requests per
+ * workgroup (16, 32, 64; also 128 when len = 67).  0 restores the tuning
+ * table / built-in choice.  cuspx_get_split reports the choice for a batch
+ * size. */
 int cuspx_set_split(int pid, int op, int lanes_per_leaf);
 int cuspx_get_split(int pid, int op, uint32_t num);
 int cuspx_get_default_param(void);

@@ -83,13 +83,11 @@ def test_split_validation_and_override():
     L = C.lib()
     for pid in range(12):
         p = C.params(pid)
-        lo = 32 if p.len > 40 else 16
         hi = 128 if p.len > 64 else 64
         good = {C.OP_SIGN: [1, 2, 4, 8, 16], C.OP_KEYGEN: [1, 2, 4, 8, 16],
-                C.OP_VERIFY: [g for g in (16, 32, 64, 128) if lo <= g <= hi] +
-                [1000 + g for g in (16, 32, 64) if g >= lo]}
+                C.OP_VERIFY: [g for g in (16, 32, 64, 128) if g <= hi]}
         bad = {C.OP_SIGN: [3, 32, -1, 1016], C.OP_KEYGEN: [5, 64],
-               C.OP_VERIFY: [8, 1, 256, 1128, lo // 2, 1000 + lo // 2]}
+               C.OP_VERIFY: [8, 1, 24, 256, 1016, 1032, 2 * hi]}
         try:
             for op in good:
                 for v in good[op]:
@@ -109,7 +107,7 @@ def test_tuning_table_is_read(tmp_path):
     with the largest threshold not above the launch size applies."""
     table = tmp_path / "t.txt"
     table.write_text("# test\nsha2-128f sign 0 4\nsha2-128f sign 2200 16\n"
-                     "sha2-128f verify 0 1032\nsha2-128f verify 50 16\nbogus line\n")
+                     "sha2-128f verify 0 32\nsha2-128f verify 50 16\nbogus line\n")
     code = ("import cuspx_amd as C\n"
             "L = C.lib()\n"
             "print(L.cuspx_get_split(1, 0, 1), L.cuspx_get_split(1, 0, 99),"
@@ -119,13 +117,13 @@ def test_tuning_table_is_read(tmp_path):
                          text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     # sign thresholds count subtrees: batch * d (d = 22 for SHA2-128f)
-    assert out.stdout.split() == ["4", "4", "16", "1032", "16"]
+    assert out.stdout.split() == ["4", "4", "16", "32", "16"]
 
 
 def test_autotune_table_compaction():
     from cuspx_amd.autotune import compact
     rows = [("s", "sign", 22, 8), ("s", "sign", 88, 8), ("s", "sign", 352, 4),
-            ("s", "sign", 1408, 4), ("s", "sign", 5632, 1), ("s", "verify", 1, 1032),
+            ("s", "sign", 1408, 4), ("s", "sign", 5632, 1), ("s", "verify", 1, 64),
             ("s", "verify", 4096, 16)]
     assert compact(rows) == [("s", "sign", 0, 8), ("s", "sign", 352, 4), ("s", "sign", 5632, 1),
-                             ("s", "verify", 0, 1032), ("s", "verify", 4096, 16)]
+                             ("s", "verify", 0, 64), ("s", "verify", 4096, 16)]
