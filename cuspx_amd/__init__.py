@@ -146,7 +146,8 @@ def set_split(pid, op, lanes_per_leaf):
     """Force a work split (0 = automatic / tuned): lanes per WOTS leaf of the
     hypertree kernel for OP_SIGN / OP_KEYGEN (1, 2, 4, 8, 16); mean lanes per
     request of the verify kernel for OP_VERIFY (16, 32, 64; 128 when len = 67),
-    which runs 256 / lanes requests per workgroup."""
+    which runs 256 / lanes requests per workgroup, or 1 for speculative
+    verification (batches up to 256 requests)."""
     rc = lib().cuspx_set_split(pid, op, lanes_per_leaf)
     if rc != OK:
         raise CuspxError("cuspx_set_split failed (%d)" % rc)

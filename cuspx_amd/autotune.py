@@ -5,7 +5,9 @@ ceil(len/S) chains). A small batch fills the chip only with a wider split; a
 large one runs fastest at S=1, which needs no cross-lane reduction. k_vht
 runs 256 / G requests per workgroup (G = 16 to 128 lanes per request on
 average): wider groups shorten each layer's chain phase down to the longest
-chain, narrower ones keep more requests busy per lane. The crossovers depend
+chain, narrower ones keep more requests busy per lane; G = 1 is speculative
+verification (every chain from every start first, batches up to 256). The
+crossovers depend
 on the set and on the batch, so they are measured: for every set, operation
 and batch size this times device-resident launches at each split the set
 allows and writes the winners as
@@ -68,7 +70,7 @@ def tune_set(name, batches, reps, log):
     stream.sync()
     hi = 128 if p.len > 64 else 64
     widths = {0: (1, 2, 4, 8, 16), 2: (1, 2, 4, 8, 16),
-              1: tuple(G for G in (16, 32, 64, 128) if G <= hi)}
+              1: (1,) + tuple(G for G in (16, 32, 64, 128) if G <= hi)}
     rows = []
     for op_name, op in OPS.items():
         for b in batches:

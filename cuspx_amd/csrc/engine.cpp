@@ -481,7 +481,7 @@ int cuspx_set_split(int pid, int op, int lanes_per_leaf) {
     if (op == kOpVerify) {
         const int g = lanes_per_leaf;
         const int hi = o->len > 64 ? 128 : 64;
-        if (g != 0 && !(g >= 16 && g <= hi && (g & (g - 1)) == 0)) return kErrArg;
+        if (g != 0 && g != 1 && !(g >= 16 && g <= hi && (g & (g - 1)) == 0)) return kErrArg;
     } else if ((op != kOpSign && op != kOpKeygen) ||
                (lanes_per_leaf != 0 && lanes_per_leaf != 1 && lanes_per_leaf != 2 &&
                 lanes_per_leaf != 4 && lanes_per_leaf != 8 && lanes_per_leaf != 16)) {

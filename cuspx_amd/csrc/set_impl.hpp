@@ -37,12 +37,27 @@ struct Layout {
     // while the launch would fill under an eighth of the chip: wider groups
     // shorten each layer's chain phase down to the longest chain, narrower
     // ones keep more requests per lane.
+    // G = 1 selects speculative verification (k_vspec + k_vht_spec): every
+    // chain from every start up front, then T_len and the climb per layer;
+    // 16x the chain work for half the dependent path, so batches up to
+    // kSpecMax requests only (the workspace keeps candidates for that many).
     static constexpr int vmin() { return 16; }
     static constexpr int vmax() { return P::len > 64 ? 128 : 64; }
-    static bool vvalid(int g) { return g >= vmin() && g <= vmax() && (g & (g - 1)) == 0; }
+    static constexpr uint32_t kSpecMax = 256;
+    static constexpr uint32_t kSpecDefault = 64;  // built-in choice without a table
+    static bool vvalid(int g, uint32_t n) {
+        if (g == 1) return n <= kSpecMax;
+        return g >= vmin() && g <= vmax() && (g & (g - 1)) == 0;
+    }
+    // candidate chain ends of min(n, kSpecMax) requests: monotone in n, so a
+    // workspace sized for a batch also holds every smaller batch's layout
+    static uint64_t spec_bytes(uint32_t n) {
+        return (uint64_t)(n < kSpecMax ? n : kSpecMax) * P::d * P::len * 16 * P::n;
+    }
     static int vsplit(int pid, uint32_t n) {
         const int t = tuned_split(pid, kOpVerify, n);
-        if (vvalid(t)) return t;
+        if (vvalid(t, n)) return t;
+        if (n <= kSpecDefault) return 1;
         int G = vmin();
         while (G < vmax() && (uint64_t)n * G * 8 <= kFillLanes) G *= 2;
         return G;
@@ -71,6 +86,7 @@ struct Layout {
         uint64_t b = al((uint64_t)n * sizeof(Plan));
         b += al((uint64_t)n * P::k * P::n);   // fors roots
         b += al((uint64_t)n * P::n);          // fors pk
+        if (op == kOpVerify) b += al(spec_bytes(n));              // speculative candidates
         if (op == kOpSign) {
             b += al((uint64_t)n * P::d * P::n);                    // layer roots
             b += al((uint64_t)n * P::d * P::len * 16 * P::n);      // chain capture
@@ -105,6 +121,10 @@ struct Layout {
         A.ht_parts = nullptr;
         A.ht_count = nullptr;
         A.tile_ctr = nullptr;
+        if (op == kOpVerify) {
+            A.cap = p;
+            p += al(spec_bytes(n));
+        }
         if (op == kOpSign) {
             A.roots = p;
             p += al((uint64_t)n * P::d * P::n);
@@ -262,7 +282,12 @@ struct SetImpl {
         prof_mark(s, "k_vfors");
         hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
-        if (G == 16) launch_vht<16>(A, n, s);
+        if (G == 1) {
+            hipLaunchKernelGGL((k_vspec<P>), dim3(cdiv((uint64_t)n * P::d * P::len * 8, 256)),
+                               dim3(256), 0, s, A);
+            prof_mark(s, "k_vspec");
+            hipLaunchKernelGGL((k_vht_spec<P>), dim3(n), dim3(64), 0, s, A);
+        } else if (G == 16) launch_vht<16>(A, n, s);
         else if (G == 32) launch_vht<32>(A, n, s);
         else if (G == 64) launch_vht<64>(A, n, s);
         else if constexpr (Lay::vmax() >= 128) launch_vht<128>(A, n, s);

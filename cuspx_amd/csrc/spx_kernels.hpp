@@ -13,7 +13,9 @@
 //           k_wots_gather lane per (request, layer, chain): picks the
 //                        captured chain value for the layer's message
 //   verify: k_vprep, k_vfors, k_fors_pk, k_vht (the chains of a workgroup's
-//           requests dealt over its lanes, layers in sequence)
+//           requests dealt over its lanes, layers in sequence); small
+//           batches: k_vspec (every chain from every start) + k_vht_spec
+//           (T_len and the climb per layer)
 //   keygen: k_kprep, k_ht in keygen mode (top layer only)
 //
 // Semantics follow sign.cu:1924-1994 (sign), sign.cu:3048-3121 (verify),
@@ -1064,6 +1066,166 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
 #pragma unroll
         for (int w = 0; w < NW; w++) ok = ok && (root_s[tid][w] == pl.root[w]);
         A.status[req0 + tid] = ok ? 0 : -1;
+    }
+}
+
+// Speculative verify for small batches, part 1: every WOTS chain of every
+// layer from every possible start.  The start of chain j in layer L is the
+// digit d_j of layer L-1's root, known only after that layer, so the chains
+// of all 16 starts are computed up front: cap[req][L][j][d] = F_14(..F_d(sig_j))
+// (wots_pk_from_sig, wots.cu:777-790, with d = 15 the signature value
+// itself).  Lane p of a chain walks the starts p and 15 - p, 15 F steps for
+// every lane.  The per-layer pass that remains (k_vht_spec) is T_len and the
+// climb only.  SHAKE256 values are stored bit-interleaved.
+template <class P>
+__global__ void __launch_bounds__(256) k_vspec(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr bool SHA2 = P::hash == kSha2;
+    constexpr int NW = P::nw;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)A.n_req * P::d * P::len * 8) return;
+    const int p = (int)(g & 7);
+    const uint64_t c = g >> 3;                       // (req * d + layer) * len + j
+    const int j = (int)(c % P::len);
+    const uint32_t layer = (uint32_t)((c / P::len) % P::d);
+    const uint32_t req = (uint32_t)(c / ((uint64_t)P::len * P::d));
+    const Plan& pl = A.plan[req];
+    Key key;
+    Hh::load_key(pl, key);
+    const uint64_t tree = layer_tree<P>(pl.tree, layer);
+    const uint32_t leaf = layer == 0 ? pl.leaf
+                                     : (uint32_t)((pl.tree >> ((layer - 1) * P::th)) &
+                                                  ((1u << P::th) - 1));
+    Adr wa = adr_make(layer, tree, kWots);
+    adr_keypair<P::th>(wa, leaf);
+    adr_chain(wa, j);
+    uint32_t sg[NW], v[NW];
+    load_val<P::hash, NW>(A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
+                              (size_t)layer * P::layer_bytes + (size_t)j * P::n,
+                          sg);
+    typename std::conditional<SHA2, int, typename Shake<P::n>::ChainIl>::type ch{};
+    if constexpr (!SHA2) {
+        Hh::chain_il_begin(key, wa, ch);
+        uint32_t t[NW];
+        Hh::to_il(sg, t);
+#pragma unroll
+        for (int w = 0; w < NW; w++) sg[w] = t[w];
+    }
+    uint32_t* out = reinterpret_cast<uint32_t*>(A.cap) + c * 16 * NW;
+    if (p == 0) {  // start 15: no step
+#pragma unroll
+        for (int w = 0; w < NW; w++) out[15 * NW + w] = sg[w];
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) v[w] = sg[w];
+    uint32_t s = p;     // chain A: start p, 15 - p steps; then chain B: start 15 - p, p steps
+    bool second = false;
+#pragma unroll 1
+    for (int it = 0; it < 15; it++) {
+        if constexpr (SHA2) {
+            adr_hash(wa, s);
+            Hh::F(key, wa, v, v);
+        } else {
+            Hh::F_il(ch, s, v, v);
+        }
+        s++;
+        if (s == (uint32_t)P::w - 1) {
+            const int d = second ? 15 - p : p;
+#pragma unroll
+            for (int w = 0; w < NW; w++) out[d * NW + w] = v[w];
+            second = true;
+            s = 15 - p;
+#pragma unroll
+            for (int w = 0; w < NW; w++) v[w] = sg[w];
+        }
+    }
+}
+
+// Speculative verify, part 2: one wave per request walks the d layers
+// (sign.cu:3077-3113): lanes pick each chain's end for the layer's message
+// from the candidates of k_vspec and load the auth path, then lane 0 runs
+// T_len and compute_root (utils.cu:94-135).  The serial work per layer is
+// T_len + h' nodes instead of the longest chain + T_len + h' nodes.
+template <class P>
+__global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
+    using Hh = HashT<P>;
+    using Key = typename Hh::Key;
+    constexpr bool SHA2 = P::hash == kSha2;
+    constexpr int NW = P::nw;
+    constexpr int LEN = P::len;
+    constexpr int TH = P::th;
+    constexpr int TLW = LEN * NW;
+    constexpr int AW = TH * NW;
+    __shared__ uint32_t pk_s[TLW];    // chain ends (SHAKE256: interleaved)
+    __shared__ uint32_t auth_s[AW];   // auth path, raw words
+    __shared__ uint32_t root_s[NW];
+    const int tid = threadIdx.x;
+    const uint32_t req = blockIdx.x;  // grid = n_req
+    const Plan& pl = A.plan[req];
+    const uint8_t* sig = A.sig + (size_t)req * A.sig_stride;
+    const uint32_t* cand = reinterpret_cast<const uint32_t*>(A.cap) + (size_t)req * P::d * LEN * 16 * NW;
+    if (tid == 0) {
+        uint32_t fp[NW];
+        load_val<P::hash, NW>(A.fors_pk + (size_t)req * P::n, fp);
+#pragma unroll
+        for (int w = 0; w < NW; w++) root_s[w] = fp[w];
+    }
+    uint64_t tree = pl.tree;
+    uint32_t leaf = pl.leaf;
+#pragma unroll 1
+    for (int layer = 0; layer < P::d; layer++) {
+        __syncthreads();
+        uint32_t msg[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) msg[w] = root_s[w];
+        const uint32_t* lc = cand + (size_t)layer * LEN * 16 * NW;
+#pragma unroll 1
+        for (int j = tid; j < LEN; j += 64) {
+            const uint32_t d = chain_len<P>(msg, j);
+#pragma unroll
+            for (int w = 0; w < NW; w++) pk_s[j * NW + w] = lc[((size_t)j * 16 + d) * NW + w];
+        }
+        const uint32_t* auth = reinterpret_cast<const uint32_t*>(
+            sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes + P::wots_bytes);
+        for (int u = tid; u < AW; u += 64) auth_s[u] = auth[u];
+        __syncthreads();
+        if (tid == 0) {
+            Key key;
+            Hh::load_key(pl, key);
+            Adr pa = adr_make(layer, tree, kWotsPk);
+            adr_keypair<TH>(pa, leaf);
+            uint32_t node[NW];
+            if constexpr (SHA2) tlen_lds<P>(key.mid, pa, pk_s, node);
+            else tlen_lds_il<P>(key, pa, pk_s, node);
+            uint32_t li = leaf;
+            Adr ha = adr_make(layer, tree, kHashTree);
+#pragma unroll 1
+            for (int h = 0; h < TH; h++) {
+                uint32_t lft[NW], rgt[NW];
+                const bool odd = li & 1u;
+                li >>= 1;
+                adr_node(ha, h + 1, li);
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    const uint32_t ap = to_native<P::hash>(auth_s[h * NW + w]);
+                    lft[w] = odd ? ap : node[w];
+                    rgt[w] = odd ? node[w] : ap;
+                }
+                Hh::H(key, ha, lft, rgt, node);
+            }
+#pragma unroll
+            for (int w = 0; w < NW; w++) root_s[w] = node[w];
+        }
+        leaf = (uint32_t)(tree & ((1u << TH) - 1));
+        tree >>= TH;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        bool ok = true;
+#pragma unroll
+        for (int w = 0; w < NW; w++) ok = ok && (root_s[w] == pl.root[w]);
+        A.status[req] = ok ? 0 : -1;
     }
 }
 

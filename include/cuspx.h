@@ -43,7 +43,10 @@ int cuspx_set_default_param(int pid);
 /* Work split of set pid at every batch size.  op 0 (sign) / 2 (keygen): lanes
  * per WOTS leaf of the hypertree kernel (1, 2, 4, 8, 16).  op 1 (verify): mean
  * lanes per request of the verify kernel, which runs 256 / lanes requests per
- * workgroup (16, 32, 64; also 128 when len = 67).  0 restores the tuning
+ * workgroup (16, 32, 64; also 128 when len = 67), or 1 for speculative
+ * verification (every WOTS chain from every start computed up front, then
+ * T_len and the subtree climb per layer; batches of up to 256 requests, larger
+ * ones fall back to the table / built-in choice).  0 restores the tuning
  * table / built-in choice.  cuspx_get_split reports the choice for a batch
  * size. */
 int cuspx_set_split(int pid, int op, int lanes_per_leaf);

@@ -85,9 +85,9 @@ def test_split_validation_and_override():
         p = C.params(pid)
         hi = 128 if p.len > 64 else 64
         good = {C.OP_SIGN: [1, 2, 4, 8, 16], C.OP_KEYGEN: [1, 2, 4, 8, 16],
-                C.OP_VERIFY: [g for g in (16, 32, 64, 128) if g <= hi]}
+                C.OP_VERIFY: [1] + [g for g in (16, 32, 64, 128) if g <= hi]}
         bad = {C.OP_SIGN: [3, 32, -1, 1016], C.OP_KEYGEN: [5, 64],
-               C.OP_VERIFY: [8, 1, 24, 256, 1016, 1032, 2 * hi]}
+               C.OP_VERIFY: [8, 2, 24, 256, 1016, 1032, 2 * hi]}
         try:
             for op in good:
                 for v in good[op]:

@@ -200,9 +200,9 @@ def test_every_leaf_split_matches_oracle(name):
                                   "shake256-128f", "shake256-256f"])
 def test_every_verify_width(name):
     """k_vht at every width (256 / G requests per workgroup, their chains dealt
-    over its lanes) accepts the valid signatures and rejects one tampered byte
-    in each part of the hypertree section; six requests leave most of each
-    workgroup's request slots empty."""
+    over its lanes) and the speculative pair (G = 1) accept the valid
+    signatures and reject one tampered byte in each part of the hypertree
+    section; six requests leave most of each workgroup's request slots empty."""
     pid = C.param_id(name)
     p = O.params(pid)
     num = 6
@@ -218,13 +218,41 @@ def test_every_verify_width(name):
     bad[2, ht + (p.len - 1) * p.n] ^= 2              # layer 0, last (checksum) chain
     bad[3, ht + p.len * p.n + 1] ^= 4                # layer 0, auth path
     bad[4, ht + (p.d - 1) * layer_bytes + 7] ^= 8    # top layer, WOTS
-    widths = [G for G in (16, 32, 64, 128) if G <= (128 if p.len > 64 else 64)]
+    widths = [1] + [G for G in (16, 32, 64, 128) if G <= (128 if p.len > 64 else 64)]
     try:
         for G in widths:
             C.set_split(pid, C.OP_VERIFY, G)
             assert C.get_split(pid, C.OP_VERIFY, num) == G
             assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * num, G
             assert list(C.batch_verify(pid, pk, bad, msgs)) == [0, -1, -1, -1, -1, 0], G
+    finally:
+        C.set_split(pid, C.OP_VERIFY, 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sha2-128f", "shake256-256f"])
+def test_verify_ragged_workgroups(name):
+    """37 requests: at G = 16 two full workgroups and one with 5 of 16 request
+    slots, at G = 32 four full and one with 5 of 8, plus the speculative pair.
+    Tampered requests sit in every workgroup; each status must be exact."""
+    pid = C.param_id(name)
+    p = O.params(pid)
+    num = 37
+    seeds, msgs, opts = _inputs(pid, num, 4711 + pid, [32, 1, 77])
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    ht = p.n + p.k * (p.a + 1) * p.n
+    layer_bytes = (p.len + p.full_h // p.d) * p.n
+    bad = sig.copy()
+    want = [0] * num
+    for i, layer in ((3, 0), (15, 5), (16, p.d - 1), (20, 2), (36, 1)):
+        bad[i, ht + layer * layer_bytes + 3 * p.n + 9] ^= 0x40
+        want[i] = -1
+    try:
+        for G in (1, 16, 32):
+            C.set_split(pid, C.OP_VERIFY, G)
+            assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * num, G
+            assert list(C.batch_verify(pid, pk, bad, msgs)) == want, G
     finally:
         C.set_split(pid, C.OP_VERIFY, 0)
 
