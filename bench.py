@@ -134,7 +134,7 @@ SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
 
 
 def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_us=1000,
-                sweep=(), sweep_seconds=3.0,
+                sweep=(), sweep_seconds=3.0, long_slots=0,
                 seed=5):
     """Serving workload: 256 resident keys over SHA2-128f / SHA2-128s /
     SHAKE256-256f, Poisson arrivals at `qps`, 80% sign / 20% verify, message
@@ -200,8 +200,10 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_
         return el, bad
 
     out = {"sets": SERVE_SETS, "keys": len(keys), "mix": "80% sign / 20% verify",
-           "message_bytes": "log-uniform 16..65536", "slots": slots, "batch_window_us": window_us}
-    with C.Server(hbm_budget_bytes=32 << 30, slots=slots, batch_window_us=window_us) as srv:
+           "message_bytes": "log-uniform 16..65536", "slots": slots, "batch_window_us": window_us,
+           "long_slots": min(long_slots, slots) or slots}
+    with C.Server(hbm_budget_bytes=32 << 30, slots=slots, batch_window_us=window_us,
+                  long_slots=long_slots) as srv:
         kids = [srv.add_key(pid, sk, pk) for pid, sk, pk in keys]
         n = int(qps * seconds)
         reqs = mix(n)
@@ -366,6 +368,8 @@ def main():
     ap.add_argument("--mixed", type=int, default=8192)
     ap.add_argument("--graph-128s", type=int, default=1)
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-long-slots", type=int, default=0,
+                    help="slots open to messages over 16 KiB at once (0: all)")
     ap.add_argument("--serve-window-us", type=int, default=1000)
     ap.add_argument("--serve-sweep", default="4000,6000,8000,10000,12000",
                     help="offered rates (req/s) tried beyond --serve-qps, 3 s each")
@@ -474,7 +478,8 @@ def main():
               file=sys.stderr, flush=True)
         serve = serve_phase(C, args.serve_seconds, args.serve_qps, args.serve_saturation,
                             args.serve_single, args.serve_slots, args.serve_window_us,
-                            [float(x) for x in args.serve_sweep.split(",") if x])
+                            [float(x) for x in args.serve_sweep.split(",") if x],
+                            long_slots=args.serve_long_slots)
     kt, kn = prof.get(kname, (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
     work = kernel_work(p, B) + (fors_work(p, B) if kname == "k_sign_fused" else 0)

@@ -445,7 +445,8 @@ def device_info():
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
                 ("slots", ctypes.c_uint32), ("max_msg_bytes", ctypes.c_uint32),
-                ("batch_window_us", ctypes.c_uint32), ("max_keys_per_set", ctypes.c_uint32)]
+                ("batch_window_us", ctypes.c_uint32), ("max_keys_per_set", ctypes.c_uint32),
+                ("long_slots", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -462,7 +463,7 @@ class Server:
     (libcuspx spx_server_*)."""
 
     def __init__(self, hbm_budget_bytes=0, max_batch=0, slots=0, max_msg_bytes=0,
-                 batch_window_us=200, max_keys_per_set=0):
+                 batch_window_us=200, max_keys_per_set=0, long_slots=0):
         L = lib()
         vp = ctypes.c_void_p
         L.spx_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -476,7 +477,7 @@ class Server:
         L.spx_server_reset_stats.argtypes = [vp]
         L.spx_server_destroy.argtypes = [vp]
         cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, max_msg_bytes, batch_window_us,
-                           max_keys_per_set)
+                           max_keys_per_set, long_slots)
         h = ctypes.c_void_p()
         _check(L.spx_server_create(ctypes.byref(cfg), ctypes.byref(h)), "spx_server_create")
         self.h = h.value

@@ -6,7 +6,8 @@
 // * Requests (sign or verify, any parameter set, message 0..max_msg bytes)
 //   queue per (set, op, message-size class: <= 2 KiB, <= 16 KiB, larger) so
 //   short messages never wait for the serial hash of a long one in the same
-//   launch.  A scheduler thread keeps up to `slots` batches in
+//   launch; batches of the longest class may be held to `long_slots` slots
+//   at once, so their long serial hashes leave slots to short requests.  A scheduler thread keeps up to `slots` batches in
 //   flight, each on its own stream with its own pinned staging and device
 //   arena.  When a slot frees, the oldest queue is packed into the next
 //   batch: immediately if the GPU is idle or the batch is full, otherwise
@@ -75,7 +76,7 @@ struct Slot {
     uint64_t cap = 0;      // device arena bytes
     uint64_t hcap = 0;     // staging bytes
     bool busy = false;
-    int pid = -1, op = -1;
+    int pid = -1, op = -1, cls = 0;
     uint64_t out_off = 0;  // offset of the outputs in h / d
     std::vector<ReqP> batch;
 };
@@ -236,13 +237,17 @@ void Server::loop() {
         }
         if (progressed) cv_done.notify_all();
         Clock::time_point wake = Clock::now() + std::chrono::milliseconds(5);
+        int long_busy = 0;
+        for (auto& sl : slots) long_busy += sl.busy && sl.cls == 2;
         for (auto& sl : slots) {
             if (sl.busy) continue;
-            // the queue whose head request is oldest
+            // the queue whose head request is oldest (the long-message class
+            // only while it holds fewer than long_slots slots)
             std::deque<ReqP>* best = nullptr;
             std::tuple<int, int, int> bk;
             for (auto& kv : queues) {
                 if (kv.second.empty()) continue;
+                if (std::get<2>(kv.first) == 2 && long_busy >= (int)cfg.long_slots) continue;
                 if (!best || kv.second.front()->t_submit < best->front()->t_submit) {
                     best = &kv.second;
                     bk = kv.first;
@@ -285,6 +290,7 @@ void Server::loop() {
                 reqs.push_back(r);
             }
             if (reqs.empty()) continue;
+            sl.cls = std::get<2>(bk);
             const int rc = launch(sl, b_pid, b_op, reqs);
             if (rc) {  // fail the batch, keep serving
                 for (auto& r : reqs) {
@@ -296,6 +302,7 @@ void Server::loop() {
                 err = rc;
             } else {
                 inflight++;
+                long_busy += sl.cls == 2;
                 progressed = true;
             }
         }
@@ -319,6 +326,7 @@ int spx_server_create(const spx_server_config* cfg, spx_server** out) {
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.max_batch) s->cfg.max_batch = 4096;
     if (!s->cfg.slots) s->cfg.slots = 4;
+    if (!s->cfg.long_slots || s->cfg.long_slots > s->cfg.slots) s->cfg.long_slots = s->cfg.slots;
     if (!s->cfg.max_msg_bytes) s->cfg.max_msg_bytes = 65536;
     if (!s->cfg.max_keys_per_set) s->cfg.max_keys_per_set = 1024;
     if (!s->cfg.hbm_budget_bytes) {

@@ -188,6 +188,10 @@ typedef struct {
     uint32_t max_msg_bytes;     /* largest message accepted (0: 65536) */
     uint32_t batch_window_us;   /* longest wait for a partial batch while the GPU is busy */
     uint32_t max_keys_per_set;  /* resident keys per parameter set (0: 1024) */
+    uint32_t long_slots;        /* most slots running batches of messages over 16 KiB
+                                   at once (0: all slots); fewer trades p99 and peak
+                                   rate for p50, since short requests then never
+                                   wait behind long serial message hashes */
 } spx_server_config;
 
 typedef struct {

@@ -73,6 +73,29 @@ def test_server_alone_equals_mixed_and_oracle():
         assert s["sign_p99_us"] > 0
 
 
+def test_server_long_message_slot_cap():
+    """With long_slots = 1 of 2 slots, requests over 16 KiB queue behind one
+    another while short ones keep the other slot; every result still equals
+    the batch API's signature for the same key, message and optrand."""
+    keys = _keys()
+    rng = random.Random(77)
+    reqs = []
+    for i in range(12):
+        k = i % len(keys)
+        p = O.params(keys[k][0])
+        mlen = rng.randrange(17 << 10, 40 << 10) if i % 2 == 0 else rng.randrange(1, 3000)
+        reqs.append((k, rng.randbytes(mlen), rng.randbytes(p.n)))
+    with C.Server(hbm_budget_bytes=2 << 30, slots=2, batch_window_us=300, long_slots=1) as srv:
+        kids = [srv.add_key(pid, sk, pk) for pid, sk, pk in keys]
+        tickets = [(srv.submit_sign(kids[k], m, opt), k) for k, m, opt in reqs]
+        got = [srv.wait(t, key=kids[k]) for t, k in tickets]
+    for (k, m, opt), (st, sig) in zip(reqs, got):
+        pid, sk, pk = keys[k]
+        assert st == 0
+        ref = C.batch_sign(pid, np.frombuffer(sk, np.uint8), [m], np.frombuffer(opt, np.uint8))
+        assert sig == bytes(ref[0])
+
+
 def test_server_admission_and_cancel():
     pid = C.param_id("sha2-128f")
     p = O.params(pid)
