@@ -78,6 +78,27 @@ def test_workspace_grows_with_the_batch():
             assert sizes == sorted(sizes), (pid, op)
 
 
+def test_verify_workspace_holds_the_speculative_candidates():
+    """Verify workspaces carry the candidate chain ends of min(n, 256)
+    requests (d x len x 16 values of n bytes each), so the size steps by that
+    amount per request up to 256 requests and by far less beyond; and a
+    forced speculative split falls back above 256 requests."""
+    L = C.lib()
+    for pid in range(12):
+        p = C.params(pid)
+        cand = p.d * p.len * 16 * p.n
+        w = lambda n: C.workspace_bytes(pid, n, C.OP_VERIFY)
+        assert w(2) - w(1) >= cand
+        assert w(256) - w(255) >= cand
+        assert w(258) - w(257) < cand
+        try:
+            assert L.cuspx_set_split(pid, C.OP_VERIFY, 1) == C.OK
+            assert L.cuspx_get_split(pid, C.OP_VERIFY, 256) == 1
+            assert L.cuspx_get_split(pid, C.OP_VERIFY, 257) != 1
+        finally:
+            L.cuspx_set_split(pid, C.OP_VERIFY, 0)
+
+
 def test_split_validation_and_override():
     """cuspx_set_split accepts exactly the documented values (host logic only)."""
     L = C.lib()
