@@ -130,6 +130,46 @@ def make_inputs(C, pid, batch):
     return to(kseeds), to(opts), to(msgs)
 
 
+def host_phase(C, pid, sk, pk, m, num, reps):
+    """The reference's batched entry points on host buffers: one
+    face_mdp_crypto_sign and one face_mdp_crypto_sign_open call per rep
+    (sign.cu:5098, 6856 signatures; fixed 32-byte message stride), so the
+    rates include the PCIe copies, the host staging and each call's
+    synchronisation.  Outputs are preallocated and touched first."""
+    import ctypes
+    import numpy as np
+    p = C.params(pid)
+    L = C.lib()
+    C.set_default_param(pid)
+    sm = np.ones((num, p.sig_bytes + 32), np.uint8)
+    mo = np.ones((num, 32), np.uint8)
+    smlen, mlen = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+    def sign():
+        assert L.face_mdp_crypto_sign(ptr(sm), ctypes.byref(smlen), ptr(m), 32, ptr(sk), num) == 0
+
+    def open_():
+        assert L.face_mdp_crypto_sign_open(ptr(mo), ctypes.byref(mlen), ptr(sm), smlen.value,
+                                           ptr(pk), num) == 0
+
+    sign()
+    open_()  # grows the library's device arena and pinned staging once
+    ts, tv = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        sign()
+        ts.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        open_()
+        tv.append(time.perf_counter() - t0)
+    assert bytes(mo.reshape(-1)) == bytes(m)
+    return {"signs_per_s": round(num / min(ts), 1), "verifies_per_s": round(num / min(tv), 1),
+            "sign_call_ms": round(min(ts) * 1e3, 3), "open_call_ms": round(min(tv) * 1e3, 3),
+            "calls": "face_mdp_crypto_sign / face_mdp_crypto_sign_open on pageable host "
+                     "buffers (%d items, best of %d); PCIe-inclusive, never `value`" % (num, reps)}
+
+
 SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
 
 
@@ -367,6 +407,8 @@ def main():
     ap.add_argument("--serve-single", type=int, default=150)
     ap.add_argument("--mixed", type=int, default=8192)
     ap.add_argument("--graph-128s", type=int, default=1)
+    ap.add_argument("--host-reps", type=int, default=3,
+                    help="face_mdp_* calls on host buffers (PCIe-inclusive rate; 0 skips)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-long-slots", type=int, default=0,
                     help="slots open to messages over 16 KiB at once (0: all)")
@@ -465,6 +507,11 @@ def main():
                                   t["write_kib"] * t["write_scale"]))
     except (OSError, ValueError):
         pass
+    host = None
+    if args.host_reps > 0:
+        print("host-buffer entry points: %d reps" % args.host_reps, file=sys.stderr, flush=True)
+        host = host_phase(C, pid, d_sk.download(np.uint8), d_pk.download(np.uint8), msgs, B,
+                          args.host_reps)
     mixed = graph128s = None
     if args.mixed > 0:
         print("mixed SHAKE batch: %d requests" % args.mixed, file=sys.stderr, flush=True)
@@ -527,6 +574,7 @@ def main():
                      "frac_at_1450_ops": round(units_s / (PEAK_VOP3_T * 1e12 / SHA256_OPS_SURVEY), 4)
                      if p.hash == 0 else None},
         "cpu_baseline": cpu,
+        "host_buffers": host,
         "serve": serve,
         "mixed_shake_batch": mixed,
         "sha2_128s_graph": graph128s,

@@ -31,47 +31,73 @@ struct Aes256 {
             sbox[p] = x ^ 0x63;
         } while (p != 1);
         sbox[0] = 0x63;
+        tables();
+    }
+    // 32-bit round tables (FIPS-197 5.1.1-5.1.3 folded per column): entry x
+    // of te0 is the MixColumns column of S(x) in row 0, (2S, S, S, 3S) with
+    // row 0 in the top byte; rows 1-3 are its byte rotations
+    uint32_t te0[256], te1[256], te2[256], te3[256];
+    void tables() {
+        for (int x = 0; x < 256; x++) {
+            const uint32_t s1 = sbox[x], s2 = xt(sbox[x]), s3 = s2 ^ s1;
+            const uint32_t w = (s2 << 24) | (s1 << 16) | (s1 << 8) | s3;
+            te0[x] = w;
+            te1[x] = (w >> 8) | (w << 24);
+            te2[x] = (w >> 16) | (w << 16);
+            te3[x] = (w >> 24) | (w << 8);
+        }
     }
     static uint8_t xt(uint8_t a) { return (uint8_t)((a << 1) ^ ((a & 0x80) ? 0x1B : 0)); }
-    void encrypt(const uint8_t key[32], const uint8_t in[16], uint8_t out[16]) const {
-        uint8_t rk[240];
-        memcpy(rk, key, 32);
+    // key schedule (FIPS-197 5.2), 60 words with row 0 in the top byte
+    void expand(const uint8_t key[32], uint32_t rk[60]) const {
+        for (int i = 0; i < 8; i++)
+            rk[i] = ((uint32_t)key[4 * i] << 24) | ((uint32_t)key[4 * i + 1] << 16) |
+                    ((uint32_t)key[4 * i + 2] << 8) | key[4 * i + 3];
         uint8_t rcon = 1;
         for (int i = 8; i < 60; i++) {
-            uint8_t t[4];
-            memcpy(t, rk + 4 * (i - 1), 4);
+            uint32_t t = rk[i - 1];
             if (i % 8 == 0) {
-                const uint8_t t0 = t[0];
-                t[0] = sbox[t[1]] ^ rcon;
-                t[1] = sbox[t[2]];
-                t[2] = sbox[t[3]];
-                t[3] = sbox[t0];
+                t = (t << 8) | (t >> 24);  // RotWord
+                t = sub(t) ^ ((uint32_t)rcon << 24);
                 rcon = xt(rcon);
             } else if (i % 8 == 4) {
-                for (int j = 0; j < 4; j++) t[j] = sbox[t[j]];
+                t = sub(t);
             }
-            for (int j = 0; j < 4; j++) rk[4 * i + j] = rk[4 * (i - 8) + j] ^ t[j];
+            rk[i] = rk[i - 8] ^ t;
         }
-        uint8_t s[16];
-        for (int i = 0; i < 16; i++) s[i] = in[i] ^ rk[i];
-        for (int r = 1; r <= 14; r++) {
-            uint8_t t[16];
+    }
+    uint32_t sub(uint32_t t) const {
+        return ((uint32_t)sbox[t >> 24] << 24) | ((uint32_t)sbox[(t >> 16) & 255] << 16) |
+               ((uint32_t)sbox[(t >> 8) & 255] << 8) | sbox[t & 255];
+    }
+    void encrypt_rk(const uint32_t rk[60], const uint8_t in[16], uint8_t out[16]) const {
+        uint32_t w[4];
+        for (int c = 0; c < 4; c++)
+            w[c] = (((uint32_t)in[4 * c] << 24) | ((uint32_t)in[4 * c + 1] << 16) |
+                    ((uint32_t)in[4 * c + 2] << 8) | in[4 * c + 3]) ^ rk[c];
+        for (int r = 1; r < 14; r++) {  // SubBytes, ShiftRows, MixColumns, AddRoundKey
+            uint32_t t[4];
             for (int c = 0; c < 4; c++)
-                for (int row = 0; row < 4; row++) t[4 * c + row] = sbox[s[4 * ((c + row) & 3) + row]];
-            if (r != 14) {
-                for (int c = 0; c < 4; c++) {
-                    uint8_t* col = t + 4 * c;
-                    const uint8_t a0 = col[0], a1 = col[1], a2 = col[2], a3 = col[3];
-                    const uint8_t all = a0 ^ a1 ^ a2 ^ a3;
-                    col[0] = a0 ^ all ^ xt(a0 ^ a1);
-                    col[1] = a1 ^ all ^ xt(a1 ^ a2);
-                    col[2] = a2 ^ all ^ xt(a2 ^ a3);
-                    col[3] = a3 ^ all ^ xt(a3 ^ a0);
-                }
-            }
-            for (int i = 0; i < 16; i++) s[i] = t[i] ^ rk[16 * r + i];
+                t[c] = te0[w[c] >> 24] ^ te1[(w[(c + 1) & 3] >> 16) & 255] ^
+                       te2[(w[(c + 2) & 3] >> 8) & 255] ^ te3[w[(c + 3) & 3] & 255] ^ rk[4 * r + c];
+            for (int c = 0; c < 4; c++) w[c] = t[c];
         }
-        memcpy(out, s, 16);
+        for (int c = 0; c < 4; c++) {  // last round: no MixColumns
+            const uint32_t t = ((uint32_t)sbox[w[c] >> 24] << 24) |
+                               ((uint32_t)sbox[(w[(c + 1) & 3] >> 16) & 255] << 16) |
+                               ((uint32_t)sbox[(w[(c + 2) & 3] >> 8) & 255] << 8) |
+                               sbox[w[(c + 3) & 3] & 255];
+            const uint32_t o = t ^ rk[56 + c];
+            out[4 * c] = (uint8_t)(o >> 24);
+            out[4 * c + 1] = (uint8_t)(o >> 16);
+            out[4 * c + 2] = (uint8_t)(o >> 8);
+            out[4 * c + 3] = (uint8_t)o;
+        }
+    }
+    void encrypt(const uint8_t key[32], const uint8_t in[16], uint8_t out[16]) const {
+        uint32_t rk[60];
+        expand(key, rk);
+        encrypt_rk(rk, in, out);
     }
 };
 
@@ -95,11 +121,12 @@ struct Drbg {
             }
         }
     }
-    void update(const uint8_t* provided) {
+    // AES_CTR_DRBG_Update (rng.cu:217-241) with the current key's schedule
+    void update(const uint32_t rk[60], const uint8_t* provided) {
         uint8_t temp[48];
         for (int i = 0; i < 3; i++) {
             inc(v);
-            aes().encrypt(key, v, temp + 16 * i);
+            aes().encrypt_rk(rk, v, temp + 16 * i);
         }
         if (provided)
             for (int i = 0; i < 48; i++) temp[i] ^= provided[i];
@@ -128,7 +155,9 @@ void drbg_init(const uint8_t entropy[48], const uint8_t* pers) {
         for (int i = 0; i < 48; i++) seed[i] ^= pers[i];
     memset(d.key, 0, 32);
     memset(d.v, 0, 16);
-    d.update(seed);
+    uint32_t rk[60];
+    aes().expand(d.key, rk);
+    d.update(rk, seed);
     d.reseed_counter = 1;
 }
 
@@ -136,16 +165,18 @@ void drbg_bytes(uint8_t* x, uint64_t xlen) {
     Drbg& d = drbg();
     std::lock_guard<std::mutex> lk(d.mu);
     uint8_t block[16];
+    uint32_t rk[60];  // the key is fixed until the update below: expand it once
+    aes().expand(d.key, rk);
     uint64_t i = 0;
     while (xlen > 0) {
         Drbg::inc(d.v);
-        aes().encrypt(d.key, d.v, block);
+        aes().encrypt_rk(rk, d.v, block);
         const uint64_t take = xlen > 15 ? 16 : xlen;
         memcpy(x + i, block, take);
         i += take;
         xlen -= take;
     }
-    d.update(nullptr);
+    d.update(rk, nullptr);
     d.reseed_counter++;
 }
 
