@@ -393,55 +393,6 @@ struct Shake {
         one_block<0>(k.sk, a, nullptr, out);
     }
 
-    // Streaming T_l.  Items are parked in a per-lane global buffer (layout
-    // [word][lane] for coalescing) and absorbed in one pass by finish(), so
-    // the 50-word sponge is not live across the WOTS chain loop.
-    struct Stream {
-        const Key* key;
-        Adr adr;
-        uint32_t* buf;
-        uint32_t stride;  // lanes in the buffer
-
-        __device__ __forceinline__ void begin(const Key& k, const Adr& a, uint32_t* lane_buf,
-                                              uint32_t lane_stride) {
-            key = &k;
-            adr = a;
-            buf = lane_buf;
-            stride = lane_stride;
-        }
-        __device__ __forceinline__ void item(int j, const uint32_t v[NW]) {
-#pragma unroll
-            for (int i = 0; i < NW; i++) buf[(size_t)(j * NW + i) * stride] = v[i];
-        }
-        template <int ITEMS>
-        __device__ __forceinline__ void finish(uint32_t out[NW]) {
-            constexpr int D = ITEMS * NW;            // data words
-            constexpr int T = NW + 8 + D;            // absorbed words
-            constexpr int NB = T / 34 + 1;           // permutations (pad always fits)
-            constexpr int TAIL = T - 34 * (NB - 1);  // words in the last block
-            uint32_t s[50];
-#pragma unroll
-            for (int i = 0; i < 50; i++) s[i] = 0;
-#pragma unroll
-            for (int i = 0; i < NW + 8; i++) s[i] = i < NW ? key->pub[i] : aw(adr, i - NW);
-            int q = NW + 8;
-#pragma unroll 1
-            for (int u = 0; u < D; u++) {
-                xor_at(s, q, buf[(size_t)u * stride]);
-                q++;
-                if (q == 34) {
-                    keccak_f(s);
-                    q = 0;
-                }
-            }
-            s[TAIL] ^= 0x1Fu;
-            s[33] ^= 0x80000000u;
-            keccak_f(s);
-#pragma unroll
-            for (int i = 0; i < NW; i++) out[i] = s[i];
-        }
-    };
-
     // ------------------------------------------------ per-request hashing
 
     // SHAKE256(pre || m) squeezed to OUTB bytes (OUTB <= 136).  m is 4-byte

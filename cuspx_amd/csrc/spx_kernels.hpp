@@ -344,6 +344,148 @@ __global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
     fors_tile<P, J>(A, blockIdx.x);
 }
 
+// thash (SHA-256) over M payload words after the 22-byte address,
+// thash_sha256.cu:16-34: one compression per loop step; rd(i) returns payload
+// word i (native order) from LDS or global memory.  The words of a block are
+// read together (one memory latency per block), and blocks of payload words
+// only cost one v_alignbit per word.
+template <int NW, int M, class Rd>
+__device__ __forceinline__ void thash_sha2_rd(const uint32_t mid[8], const Adr& a, const Rd& rd,
+                                              uint32_t out[NW]) {
+    constexpr int B = (6 + M + 2 + 15) / 16;             // compression blocks
+    constexpr int FT = (M + 5) / 16;                     // first block that is not payload only
+    constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+    static_assert(FT >= 1 && FT < B, "a head, payload and tail blocks");
+    uint32_t st[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) st[i] = mid[i];
+#pragma unroll 1
+    for (int blk = 0; blk < B; blk++) {
+        uint32_t w[16], x[17];
+        if (blk == 0) {
+#pragma unroll
+            for (int j = 0; j < 11; j++) x[j] = rd(j);
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                w[i] = i < 5 ? a.w[i < 5 ? i : 0]
+                     : i == 5 ? (a.w[5] | (x[0] >> 16))
+                              : __builtin_amdgcn_alignbit(x[i - 6 > 0 ? i - 6 : 0], x[i - 5], 16);
+        } else if (blk < FT) {
+            const int base = 16 * blk - 6;
+#pragma unroll
+            for (int j = 0; j < 17; j++) x[j] = rd(base + j);
+#pragma unroll
+            for (int i = 0; i < 16; i++) w[i] = __builtin_amdgcn_alignbit(x[i], x[i + 1], 16);
+        } else {
+#pragma unroll
+            for (int tb = FT; tb < B; tb++) {
+                if (blk == tb) {
+#pragma unroll
+                    for (int j = 0; j < 17; j++) {
+                        const int k = 16 * tb - 6 + j;
+                        x[j] = rd(k < M ? k : M - 1);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const int g = 16 * tb + i;
+                        uint32_t y = 0;
+                        if (g < 5 + M) y = __builtin_amdgcn_alignbit(x[i], x[i + 1], 16);
+                        else if (g == 5 + M) y = (x[i] << 16) | 0x8000u;
+                        else if (g == 16 * B - 1) y = bits;
+                        w[i] = y;
+                    }
+                }
+            }
+        }
+        sha256_compress(st, w);
+    }
+#pragma unroll
+    for (int i = 0; i < NW; i++) out[i] = st[i];
+}
+
+// SHAKE256 thash over M payload words (thash_shake256.cu:15-25: pub_seed ||
+// addr32 || payload), 136 bytes per permutation; rd(i) returns payload word
+// i, read a block at a time.  IL: the payload words are bit-interleaved and
+// the sponge runs in the interleaved domain (keccak_f_il); else plain.  The
+// output is a plain value either way.
+template <class P, int M, bool IL, class Rd>
+__device__ __forceinline__ void thash_shake_rd(const typename Shake<P::n>::Key& key, const Adr& a,
+                                               const Rd& rd, uint32_t out[P::nw]) {
+    using Hh = Shake<P::n>;
+    constexpr int NW = P::nw;
+    constexpr int T = NW + 8 + M;             // absorbed words
+    constexpr int NB = T / 34 + 1;            // permutations (the pad always fits)
+    constexpr int TAIL = T - 34 * (NB - 1);   // words in the last block: the 0x1F byte follows
+    static_assert(NB >= 2, "at least two blocks");
+    uint32_t pre[NW + 8];                     // pub_seed || address
+    if constexpr (IL) {
+        typename Hh::ChainIl c;
+        Hh::chain_il_begin(key, a, c);
+#pragma unroll
+        for (int i = 0; i < NW + 8; i++) pre[i] = c.w[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NW + 8; i++) pre[i] = i < NW ? key.pub[i < NW ? i : 0] : Hh::aw(a, i - NW);
+    }
+    uint32_t s[50];
+#pragma unroll
+    for (int i = 0; i < 50; i++) s[i] = 0;
+#pragma unroll 1
+    for (int blk = 0; blk < NB; blk++) {
+        if (blk == 0) {
+            uint32_t x[34 - NW - 8];
+#pragma unroll
+            for (int j = 0; j < 34 - NW - 8; j++) x[j] = rd(j);
+#pragma unroll
+            for (int i = 0; i < 34; i++) s[i] ^= i < NW + 8 ? pre[i < NW + 8 ? i : 0] : x[i - NW - 8];
+        } else if (blk < NB - 1) {
+            const int base = 34 * blk - NW - 8;
+            uint32_t x[34];
+#pragma unroll
+            for (int j = 0; j < 34; j++) x[j] = rd(base + j);
+#pragma unroll
+            for (int i = 0; i < 34; i++) s[i] ^= x[i];
+        } else {
+            constexpr int base = 34 * (NB - 1) - NW - 8;
+            uint32_t x[TAIL > 0 ? TAIL : 1];
+#pragma unroll
+            for (int j = 0; j < TAIL; j++) x[j] = rd(base + j);
+#pragma unroll
+            for (int i = 0; i < TAIL; i++) s[i] ^= x[i];
+            if constexpr (!IL) {
+                s[TAIL] ^= 0x1Fu;
+            } else if constexpr ((TAIL & 1) == 0) {  // 0x1F at bit 0 of lane TAIL / 2
+                s[TAIL] ^= 7u;
+                s[TAIL + 1] ^= 3u;
+            } else {                                  // 0x1F at bit 32 of lane TAIL / 2
+                s[TAIL - 1] ^= 7u << 16;
+                s[TAIL] ^= 3u << 16;
+            }
+            s[33] ^= 0x80000000u;  // 0x80 at byte 135 (lane 16, bit 63: odd, so the same word)
+        }
+        if constexpr (IL) keccak_f_il(s);
+        else keccak_f(s);
+    }
+    if constexpr (IL) Hh::from_il(s, out);
+    else {
+#pragma unroll
+        for (int i = 0; i < NW; i++) out[i] = s[i];
+    }
+}
+
+// T_len of one WOTS public key held in LDS: native words (SHA-256), or
+// bit-interleaved words (SHAKE256)
+template <class P>
+__device__ __forceinline__ void tlen_lds(const uint32_t mid[8], const Adr& a, const uint32_t* pk,
+                                         uint32_t out[P::nw]) {
+    thash_sha2_rd<P::nw, P::len * P::nw>(mid, a, [&](int i) { return pk[i]; }, out);
+}
+template <class P>
+__device__ __forceinline__ void tlen_lds_il(const typename Shake<P::n>::Key& key, const Adr& a,
+                                            const uint32_t* pk, uint32_t out[P::nw]) {
+    thash_shake_rd<P, P::len * P::nw, true>(key, a, [&](int i) { return pk[i]; }, out);
+}
+
 // FORS public key: T_k over the k roots (fors.cu:142), lane per request
 template <class P>
 __global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
@@ -357,18 +499,12 @@ __global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
     Hh::load_key(pl, key);
     Adr adr = adr_make(0, pl.tree, kForsPk);
     adr_keypair<P::th>(adr, pl.leaf);
-    const uint8_t* roots = A.fors_roots + (size_t)i * P::k * P::n;
+    const uint32_t* rw = reinterpret_cast<const uint32_t*>(A.fors_roots + (size_t)i * P::k * P::n);
     uint32_t out[NW];
-    typename Hh::Stream st;
-    if constexpr (P::hash == kSha2) {
-        st.begin(key, adr);
-        const uint32_t* rw = reinterpret_cast<const uint32_t*>(roots);
-#pragma unroll 1
-        for (int u = 0; u < P::k * NW; u++) st.word(bswap32(rw[u]));
-    } else {
-        st.begin(key, adr, reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(roots)), 1);
-    }
-    st.template finish<P::k * (P::hash == kSha2 ? NW : 1)>(out);
+    if constexpr (P::hash == kSha2)
+        thash_sha2_rd<NW, P::k * NW>(key.mid, adr, [&](int u) { return bswap32(rw[u]); }, out);
+    else
+        thash_shake_rd<P, P::k * NW, false>(key, adr, [&](int u) { return rw[u]; }, out);
     store_val<P::hash, NW>(A.fors_pk + (size_t)i * P::n, out);
 }
 
@@ -553,12 +689,11 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     adr_keypair<P::th>(wa, u);
     Adr pa = adr_make(layer, tree, kWotsPk);
     adr_keypair<P::th>(pa, u);
-    typename Hh::Stream ts;
-    if constexpr (P::hash == kSha2) {
-        ts.begin(key, pa);
-    } else {
-        ts.begin(key, pa, A.tl_buf + leaf_g, A.tl_lanes);
-    }
+    // SHA-256 at S = 1 streams T_len in registers; otherwise the chain ends
+    // are parked in tl_buf and one lane per leaf absorbs them afterwards
+    typename Sha2<P::n>::Stream ts;
+    if constexpr (P::hash == kSha2 && !Sh::park) ts.begin(key, pa);
+    auto parked = [&](int w) { return A.tl_buf[(size_t)w * A.tl_lanes + leaf_g]; };
 #pragma unroll 1
     for (int j = q; j < P::len; j += S) {
         uint32_t v[NW];
@@ -586,7 +721,8 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
                 Hh::F_il(ch, s, vi, vi);
                 if (capture) store_val<P::hash, NW>(cap + (size_t)(j * 16 + s + 1) * P::n, vi);
             }
-            Hh::from_il(vi, v);
+#pragma unroll
+            for (int w = 0; w < NW; w++) v[w] = vi[w];  // parked interleaved
         }
         if constexpr (!Sh::park) {
 #pragma unroll 1
@@ -600,14 +736,10 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     uint32_t node[NW];
     if (q == 0) {
         if constexpr (P::hash == kSha2) {
-            if constexpr (Sh::park) {
-#pragma unroll 1
-                for (int w = 0; w < P::len * NW; w++)
-                    ts.word(A.tl_buf[(size_t)w * A.tl_lanes + leaf_g]);
-            }
-            ts.template finish<P::len * NW>(node);
+            if constexpr (Sh::park) thash_sha2_rd<NW, P::len * NW>(key.mid, pa, parked, node);
+            else ts.template finish<P::len * NW>(node);
         } else {
-            ts.template finish<P::len>(node);
+            thash_shake_rd<P, P::len * NW, true>(key, pa, parked, node);
         }
         uint8_t* auth = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
                         (size_t)layer * P::layer_bytes + P::wots_bytes;
@@ -736,101 +868,6 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
 #pragma unroll
         for (int w = 0; w < NW; w++) dst[w] = src[w];
     }
-}
-
-// T_len of one WOTS public key held in LDS as native words (thash with the
-// WOTS_PK address, thash_sha256.cu:16-34): one compression per iteration;
-// blocks of payload words only cost one v_alignbit per word.
-template <class P>
-__device__ __forceinline__ void tlen_lds(const uint32_t mid[8], const Adr& a, const uint32_t* pk,
-                                         uint32_t out[P::nw]) {
-    constexpr int M = P::len * P::nw;
-    constexpr int B = (6 + M + 2 + 15) / 16;             // compression blocks
-    constexpr int FT = (5 + M - 15 + 15) / 16;           // first block that is not payload only
-    constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
-    static_assert(FT >= 1 && FT < B, "T_len spans a head, payload and tail blocks");
-    uint32_t st[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) st[i] = mid[i];
-#pragma unroll 1
-    for (int blk = 0; blk < B; blk++) {
-        uint32_t w[16];
-        if (blk == 0) {
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                w[i] = i < 5 ? a.w[i < 5 ? i : 0]
-                     : i == 5 ? (a.w[5] | (pk[0] >> 16))
-                              : __builtin_amdgcn_alignbit(pk[i - 6 > 0 ? i - 6 : 0], pk[i - 5], 16);
-        } else if (blk < FT) {
-            const uint32_t* p = pk + 16 * blk - 6;
-#pragma unroll
-            for (int i = 0; i < 16; i++) w[i] = __builtin_amdgcn_alignbit(p[i], p[i + 1], 16);
-        } else {
-#pragma unroll
-            for (int tb = FT; tb < B; tb++) {
-                if (blk == tb) {
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        const int g = 16 * tb + i;
-                        uint32_t x = 0;
-                        if (g < 5 + M)
-                            x = __builtin_amdgcn_alignbit(pk[g - 6], pk[g - 5 < M ? g - 5 : M - 1], 16);
-                        else if (g == 5 + M) x = (pk[M - 1] << 16) | 0x8000u;
-                        else if (g == 16 * B - 1) x = bits;
-                        w[i] = x;
-                    }
-                }
-            }
-        }
-        sha256_compress(st, w);
-    }
-#pragma unroll
-    for (int i = 0; i < P::nw; i++) out[i] = st[i];
-}
-
-// SHAKE256 T_len of one WOTS public key held in LDS as bit-interleaved words
-// (thash_shake256.cu:15-25: pub_seed || addr32 || pk), absorbed 136 bytes per
-// permutation in the interleaved domain; the output is a plain value.
-template <class P>
-__device__ __forceinline__ void tlen_lds_il(const typename Shake<P::n>::Key& key, const Adr& a,
-                                            const uint32_t* pk, uint32_t out[P::nw]) {
-    using Hh = Shake<P::n>;
-    constexpr int NW = P::nw;
-    constexpr int M = P::len * NW;
-    constexpr int T = NW + 8 + M;             // absorbed words
-    constexpr int NB = T / 34 + 1;            // permutations (the pad always fits)
-    constexpr int TAIL = T - 34 * (NB - 1);   // words in the last block: the 0x1F byte follows
-    static_assert(NB >= 2, "T_len spans at least two blocks");
-    typename Hh::ChainIl pre;                 // pub_seed || address, interleaved
-    Hh::chain_il_begin(key, a, pre);
-    uint32_t s[50];
-#pragma unroll
-    for (int i = 0; i < 50; i++) s[i] = 0;
-#pragma unroll 1
-    for (int blk = 0; blk < NB; blk++) {
-        if (blk == 0) {
-#pragma unroll
-            for (int i = 0; i < 34; i++) s[i] ^= i < NW + 8 ? pre.w[i < NW + 8 ? i : 0] : pk[i - NW - 8];
-        } else if (blk < NB - 1) {
-            const uint32_t* p = pk + 34 * blk - NW - 8;
-#pragma unroll
-            for (int i = 0; i < 34; i++) s[i] ^= p[i];
-        } else {
-            constexpr int base = 34 * (NB - 1) - NW - 8;
-#pragma unroll
-            for (int i = 0; i < TAIL; i++) s[i] ^= pk[base + i];
-            if constexpr ((TAIL & 1) == 0) {  // 0x1F at bit 0 of lane TAIL / 2
-                s[TAIL] ^= 7u;
-                s[TAIL + 1] ^= 3u;
-            } else {                          // 0x1F at bit 32 of lane TAIL / 2
-                s[TAIL - 1] ^= 7u << 16;
-                s[TAIL] ^= 3u << 16;
-            }
-            s[33] ^= 0x80000000u;             // 0x80 at byte 135: lane 16, bit 63
-        }
-        keccak_f_il(s);
-    }
-    Hh::from_il(s, out);
 }
 
 // verify: the d layers of R = 256 / G requests per workgroup in order
