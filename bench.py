@@ -174,7 +174,7 @@ SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
 
 
 def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_us=1000,
-                sweep=(), sweep_seconds=3.0, long_slots=0,
+                sweep=(), sweep_seconds=5.0, long_slots=0,
                 seed=5):
     """Serving workload: 256 resident keys over SHA2-128f / SHA2-128s /
     SHAKE256-256f, Poisson arrivals at `qps`, 80% sign / 20% verify, message
@@ -414,7 +414,7 @@ def main():
                     help="slots open to messages over 16 KiB at once (0: all)")
     ap.add_argument("--serve-window-us", type=int, default=1000)
     ap.add_argument("--serve-sweep", default="4000,6000,8000,10000,12000",
-                    help="offered rates (req/s) tried beyond --serve-qps, 3 s each")
+                    help="offered rates (req/s) tried beyond --serve-qps, 5 s each")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
