@@ -196,8 +196,7 @@ def test_every_leaf_split_matches_oracle(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["sha2-128f", "shake256-192f", "sha2-256f", "sha2-128s",
-                                  "shake256-128f", "shake256-256f"])
+@pytest.mark.parametrize("name", F_SETS + S_SETS)
 def test_every_verify_width(name):
     """k_vht at every width (256 / G requests per workgroup, their chains dealt
     over its lanes) and the speculative pair (G = 1) accept the valid
