@@ -45,6 +45,8 @@ struct Layout {
     static constexpr int vmax() { return P::len > 64 ? 128 : 64; }
     static constexpr uint32_t kSpecMax = 256;
     static constexpr uint32_t kSpecDefault = 64;  // built-in choice without a table
+    // sign batches up to this size run T_k inside the FORS tiles
+    static constexpr uint32_t kForsPkInline = 256;
     static bool vvalid(int g, uint32_t n) {
         if (g == 1) return n <= kSpecMax;
         return g >= vmin() && g <= vmax() && (g & (g - 1)) == 0;
@@ -86,6 +88,7 @@ struct Layout {
         uint64_t b = al((uint64_t)n * sizeof(Plan));
         b += al((uint64_t)n * P::k * P::n);   // fors roots
         b += al((uint64_t)n * P::n);          // fors pk
+        if (op == kOpSign) b += al((uint64_t)n * 4);  // fors root counters
         if (op == kOpVerify) b += al(spec_bytes(n));              // speculative candidates
         if (op == kOpSign) {
             b += al((uint64_t)n * P::d * P::n);                    // layer roots
@@ -114,6 +117,12 @@ struct Layout {
         p += al((uint64_t)n * P::k * P::n);
         A.fors_pk = p;
         p += al((uint64_t)n * P::n);
+        A.fors_count = nullptr;
+        if (op == kOpSign) {
+            // T_k in the FORS tiles for small batches only (see fors_root_done)
+            if (n <= kForsPkInline) A.fors_count = (uint32_t*)p;
+            p += al((uint64_t)n * 4);
+        }
         A.roots = nullptr;
         A.cap = nullptr;
         A.tl_buf = nullptr;
@@ -239,16 +248,22 @@ struct SetImpl {
         prof_mark(s, "k_prep");
         // one persistent launch for FORS + hypertree tiles: FORS then overlaps
         // the hypertree instead of running before it
+        // (small batches: T_k runs on the lane that stores a request's last
+        // FORS root, inside the FORS tiles)
         if (launch_fused(A, n, S, s)) {
             prof_mark(s, "k_sign_fused");
-            hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
-            prof_mark(s, "k_fors_pk");
+            if (!A.fors_count) {
+                hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+                prof_mark(s, "k_fors_pk");
+            }
         } else {
             hipLaunchKernelGGL((k_fors<P, J>), dim3(cdiv((uint64_t)n * P::k, fors_tpw)), dim3(256),
                                0, s, A);
             prof_mark(s, "k_fors");
-            hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
-            prof_mark(s, "k_fors_pk");
+            if (!A.fors_count) {
+                hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+                prof_mark(s, "k_fors_pk");
+            }
             launch_ht<0>(A, n * P::d, S, s);
             prof_mark(s, "k_ht");
         }

@@ -5,7 +5,8 @@
 //
 //   sign:   k_prep       lane per request: midstate, PRF_msg, H_msg, plan
 //           k_fors       L lanes per FORS tree (2^J leaves per lane), LDS reduce
-//           k_fors_pk    lane per request: T_k over the k roots
+//           k_fors_pk    lane per request: T_k over the k roots (small
+//                        batches: run by the lane storing the last root)
 //           k_ht         lane per WOTS leaf of every hypertree layer (all d
 //                        layers at once: every layer's tree/leaf comes from
 //                        H_msg alone), LDS reduce, auth path, layer root;
@@ -65,6 +66,7 @@ struct BatchArgs {
     uint32_t* ht_parts;       // split subtrees: part roots [tree][part][nw]
     uint32_t* ht_count;       // split subtrees: parts finished per tree (zeroed by the prep kernel)
     uint32_t* tile_ctr;       // fused signing: next tile (zeroed by the prep kernel)
+    uint32_t* fors_count;     // FORS roots finished per request (zeroed by the prep kernels)
     uint32_t n_req;
 };
 
@@ -144,6 +146,7 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
         for (int layer = 0; layer < P::d; layer++) A.ht_count[(size_t)i * P::d + layer] = 0;
     }
     if (A.tile_ctr && i == 0) *A.tile_ctr = 0;
+    if (A.fors_count) A.fors_count[i] = 0;
 }
 
 // verify: midstate, digest = H_msg(R from the signature), plan
@@ -229,6 +232,26 @@ __global__ void __launch_bounds__(64) k_kprep(BatchArgs A) {
 }
 
 // ------------------------------------------------------------------ FORS
+
+template <class P>
+__device__ __forceinline__ void fors_pk_of(const BatchArgs& A, uint32_t i);
+
+// a FORS root of request req is stored.  Small sign batches (fors_count
+// set): count it, and the lane that stores the request's last root computes
+// the FORS public key, so T_k overlaps the hypertree tiles instead of taking
+// a launch of its own (the k roots of one request come from several
+// workgroups; every counter was zeroed by the prep kernel).  Large batches
+// launch k_fors_pk instead: there the per-tile release fences (an L2
+// write-back each) cost more than the launch.
+template <class P>
+__device__ __forceinline__ void fors_root_done(const BatchArgs& A, uint32_t req) {
+    if (!A.fors_count) return;
+    __threadfence();  // release this root before counting it
+    if (atomicAdd(&A.fors_count[req], 1u) == (uint32_t)P::k - 1) {
+        __threadfence();  // acquire: every root of the request is visible
+        fors_pk_of<P>(A, req);
+    }
+}
 
 // sign: FORS trees, L = 2^(a-J) lanes per tree, 2^J leaves per lane; `blk`
 // is the workgroup's tile (TPW trees), so the same body serves k_fors and the
@@ -332,8 +355,10 @@ __device__ __forceinline__ void fors_tile(const BatchArgs& A, uint32_t blk) {
                 const uint32_t tg = p2.fors_idx[t2];
                 if (hgt < (uint32_t)P::a && x == ((tg >> hgt) ^ 1u))
                     store_val<P::hash, NW>(rec2 + P::n + (size_t)hgt * P::n, out);
-                if (hgt == (uint32_t)P::a)
+                if (hgt == (uint32_t)P::a) {
                     store_val<P::hash, NW>(A.fors_roots + ((size_t)req2 * P::k + t2) * P::n, out);
+                    fors_root_done<P>(A, req2);
+                }
             }
         }
     }
@@ -486,14 +511,13 @@ __device__ __forceinline__ void tlen_lds_il(const typename Shake<P::n>::Key& key
     thash_shake_rd<P, P::len * P::nw, true>(key, a, [&](int i) { return pk[i]; }, out);
 }
 
-// FORS public key: T_k over the k roots (fors.cu:142), lane per request
+// FORS public key of request i: T_k over its k roots (fors.cu:142); the
+// roots are read past the caches, as another workgroup may have stored them
 template <class P>
-__global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
+__device__ __forceinline__ void fors_pk_of(const BatchArgs& A, uint32_t i) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     constexpr int NW = P::nw;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n_req) return;
     const Plan& pl = A.plan[i];
     Key key;
     Hh::load_key(pl, key);
@@ -502,10 +526,20 @@ __global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
     const uint32_t* rw = reinterpret_cast<const uint32_t*>(A.fors_roots + (size_t)i * P::k * P::n);
     uint32_t out[NW];
     if constexpr (P::hash == kSha2)
-        thash_sha2_rd<NW, P::k * NW>(key.mid, adr, [&](int u) { return bswap32(rw[u]); }, out);
+        thash_sha2_rd<NW, P::k * NW>(
+            key.mid, adr, [&](int u) { return bswap32(__builtin_nontemporal_load(rw + u)); }, out);
     else
-        thash_shake_rd<P, P::k * NW, false>(key, adr, [&](int u) { return rw[u]; }, out);
+        thash_shake_rd<P, P::k * NW, false>(
+            key, adr, [&](int u) { return __builtin_nontemporal_load(rw + u); }, out);
     store_val<P::hash, NW>(A.fors_pk + (size_t)i * P::n, out);
+}
+
+// FORS public keys of a whole batch, lane per request (large sign batches
+// and verify)
+template <class P>
+__global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n_req) fors_pk_of<P>(A, i);
 }
 
 // verify: one FORS tree per lane: leaf from the revealed sk, climb with the
