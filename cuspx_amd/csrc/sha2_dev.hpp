@@ -70,6 +70,76 @@ __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) 
     st[7] += h;
 }
 
+// ------------------------------------------------------------ lane pairs
+//
+// One compression on a lane pair (lanes 2i and 2i+1 of a wave, both active,
+// same chaining value and message): the even lane holds e, f, g, h and the
+// odd lane a, b, c, d in s[0..3]; the message is held as word pairs v[k] =
+// W[2k] (even lane) / W[2k+1] (odd lane).  Each round is one instruction
+// stream: Sigma1(e) and Ch(e, f, g) on the even lane, Sigma0(a) and
+// Maj(a, b, c) = Ch(~(a ^ b), b, c) on the odd one (per-lane rotate counts),
+// then a quad_perm DPP swap gives the odd lane T1 and the even lane d.  The
+// schedule makes a word pair per step.  12 instructions per round and 14
+// per schedule pair against 14 and 20 on one lane: for serial hashes whose
+// lane would otherwise run alone (T_len and Merkle climbs in verification).
+
+// the other lane of the pair (DPP quad_perm [1, 0, 3, 2])
+__device__ __forceinline__ uint32_t pair_swap(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
+}
+
+// m ? a : b bitwise, for a lane mask m.  Written as a bit select so that a
+// choice between two elements of one array never becomes an indexed read
+// (which the compiler lowers to a compare-and-select chain over the array).
+__device__ __forceinline__ uint32_t lane_sel(uint32_t m, uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
+}
+
+__device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32_t vin[8]) {
+    const bool odd = threadIdx.x & 1u;
+    const uint32_t m = odd ? ~0u : 0u;
+    const uint32_t r1 = odd ? 2u : 6u, r2 = odd ? 13u : 11u, r3 = odd ? 22u : 25u;
+    uint32_t v[8], z[8];  // pairs k - 8 .. k - 1 (index k & 7) and their swaps
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        v[k] = vin[k];
+        z[k] = pair_swap(vin[k]);
+    }
+    uint32_t q0 = s[0], q1 = s[1], q2 = s[2], q3 = s[3];
+#pragma unroll
+    for (int t = 0; t < 64; t++) {
+        if (t >= 16 && (t & 1) == 0) {
+            // W[2k] = s1(W[2k-2]) + W[2k-7] + s0(W[2k-15]) + W[2k-16] (even lane)
+            // W[2k+1] = s1(W[2k-1]) + W[2k-6] + s0(W[2k-14]) + W[2k-15] (odd lane)
+            const int k = t >> 1;
+            const uint32_t y = v[(k - 1) & 7];
+            const uint32_t x = lane_sel(m, z[(k - 7) & 7], z[k & 7]);
+            const uint32_t w7 = lane_sel(m, z[(k - 3) & 7], z[(k - 4) & 7]);
+            const uint32_t s1 = xor3(rotr32(y, 17), rotr32(y, 19), y >> 10);
+            const uint32_t s0 = xor3(rotr32(x, 7), rotr32(x, 18), x >> 3);
+            const uint32_t nv = s1 + w7 + s0 + v[k & 7];
+            v[k & 7] = nv;
+            z[k & 7] = pair_swap(nv);
+        }
+        const uint32_t wt = (t & 1) ? z[(t >> 1) & 7] : v[(t >> 1) & 7];  // W[t] on the even lane
+        const uint32_t hk = (q3 + kSha256K[t] + wt) & ~m;                 // h + K + W | 0
+        const uint32_t sg = xor3(__builtin_amdgcn_alignbit(q0, q0, r1),
+                                 __builtin_amdgcn_alignbit(q0, q0, r2),
+                                 __builtin_amdgcn_alignbit(q0, q0, r3));
+        const uint32_t fn = bit_ch(bit_chi(q0, q1, m), q1, q2);  // Ch(e,f,g) | Maj(a,b,c)
+        const uint32_t tt = sg + fn + hk;                          // T1 | T2
+        const uint32_t nw = tt + pair_swap(lane_sel(m, q3, tt));   // d + T1 | T1 + T2
+        q3 = q2;
+        q2 = q1;
+        q1 = q0;
+        q0 = nw;
+    }
+    s[0] += q0;
+    s[1] += q1;
+    s[2] += q2;
+    s[3] += q3;
+}
+
 // write x into b[q] for a wave-uniform runtime q (v_cndmask per slot)
 __device__ __forceinline__ void put16(uint32_t b[16], int q, uint32_t x) {
 #pragma unroll

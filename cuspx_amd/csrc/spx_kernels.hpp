@@ -428,6 +428,66 @@ __device__ __forceinline__ void thash_sha2_rd(const uint32_t mid[8], const Adr& 
     for (int i = 0; i < NW; i++) out[i] = st[i];
 }
 
+// thash_sha2_rd on a lane pair (both lanes active with the same mid, a and
+// payload; sha256_compress_pair): rd2(j) returns payload word j + (lane & 1)
+// clamped to 0..M-1, so each lane reads the words of its own half of every
+// block.  The output is in both lanes.
+template <int NW, int M, class Rd2>
+__device__ __forceinline__ void thash_sha2_pair(const uint32_t mid[8], const Adr& a, const Rd2& rd2,
+                                                uint32_t out[NW]) {
+    constexpr int B = (6 + M + 2 + 15) / 16;  // compression blocks
+    constexpr int FT = (M + 5) / 16;          // first block that is not payload only
+    constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+    const bool odd = threadIdx.x & 1u;
+    const uint32_t m = odd ? ~0u : 0u;
+    // word pairs of block tb (a constant once unrolled): word g = 16 tb + i is
+    // address, payload re-aligned by 2 bytes (lo / hi = this lane's payload
+    // words g - 6 and g - 5), the 0x80 pad, zero or the bit length
+    auto block = [&](int tb, uint32_t v[8]) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t lo = rd2(16 * tb - 6 + 2 * k), hi = rd2(16 * tb - 5 + 2 * k);
+            uint32_t y[2];
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int g = 16 * tb + 2 * k + p;
+                y[p] = g < 5        ? a.w[g < 5 ? g : 0]
+                     : g == 5       ? (a.w[5] | (lo >> 16))
+                     : g < 5 + M    ? __builtin_amdgcn_alignbit(lo, hi, 16)
+                     : g == 5 + M   ? ((lo << 16) | 0x8000u)
+                     : g == 16 * B - 1 ? bits : 0u;
+            }
+            v[k] = lane_sel(m, y[1], y[0]);
+        }
+    };
+    uint32_t st[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) st[i] = lane_sel(m, mid[i], mid[4 + i]);
+#pragma unroll 1
+    for (int blk = 0; blk < B; blk++) {
+        uint32_t v[8];
+        if (blk == 0) {
+            block(0, v);
+        } else if (blk < FT) {  // payload only
+            const int base = 16 * blk - 6;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                v[k] = __builtin_amdgcn_alignbit(rd2(base + 2 * k), rd2(base + 2 * k + 1), 16);
+        } else {
+#pragma unroll
+            for (int tb = 1; tb < B; tb++)
+                if (tb >= FT && blk == tb) block(tb, v);
+        }
+        sha256_compress_pair(st, v);
+    }
+    // a..d are on the odd lane, e..h on the even one
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const uint32_t own = st[w & 3], other = pair_swap(st[w & 3]);
+        out[w] = w < 4 ? lane_sel(m, own, other) : lane_sel(m, other, own);
+    }
+}
+
 // SHAKE256 thash over M payload words (thash_shake256.cu:15-25: pub_seed ||
 // addr32 || payload), 136 bytes per permutation; rd(i) returns payload word
 // i, read a block at a time.  IL: the payload words are bit-interleaved and
@@ -498,13 +558,7 @@ __device__ __forceinline__ void thash_shake_rd(const typename Shake<P::n>::Key& 
     }
 }
 
-// T_len of one WOTS public key held in LDS: native words (SHA-256), or
-// bit-interleaved words (SHAKE256)
-template <class P>
-__device__ __forceinline__ void tlen_lds(const uint32_t mid[8], const Adr& a, const uint32_t* pk,
-                                         uint32_t out[P::nw]) {
-    thash_sha2_rd<P::nw, P::len * P::nw>(mid, a, [&](int i) { return pk[i]; }, out);
-}
+// T_len of one WOTS public key held in LDS as bit-interleaved words (SHAKE256)
 template <class P>
 __device__ __forceinline__ void tlen_lds_il(const typename Shake<P::n>::Key& key, const Adr& a,
                                             const uint32_t* pk, uint32_t out[P::nw]) {
@@ -904,6 +958,45 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     }
 }
 
+// Verification of one layer's subtree, SHA-256 on a lane pair (both lanes
+// active): T_len over the WOTS public key in LDS (native words, pk) and the
+// climb (compute_root, utils.cu:94-135) with the auth path in LDS (raw
+// words, auth); nd is NW words of LDS for the node of the climb.  node is
+// the layer root, in both lanes.
+template <class P>
+__device__ __forceinline__ void tlen_climb_pair(const uint32_t mid[8], uint32_t layer,
+                                                uint64_t tree, uint32_t leaf, const uint32_t* pk,
+                                                const uint32_t* auth, uint32_t* nd,
+                                                uint32_t node[P::nw]) {
+    constexpr int NW = P::nw;
+    constexpr int TLW = P::len * NW;
+    const int half = threadIdx.x & 1;
+    Adr pa = adr_make(layer, tree, kWotsPk);
+    adr_keypair<P::th>(pa, leaf);
+    thash_sha2_pair<NW, TLW>(mid, pa, [&](int j) {
+        const int i = j + half;
+        return pk[i < 0 ? 0 : i < TLW ? i : TLW - 1];
+    }, node);
+    uint32_t li = leaf;
+    Adr ha = adr_make(layer, tree, kHashTree);
+#pragma unroll 1
+    for (int h = 0; h < P::th; h++) {
+        const bool right = li & 1u;  // node is the right child
+        li >>= 1;
+        adr_node(ha, h + 1, li);
+        // payload node || auth or auth || node, raw words from LDS
+#pragma unroll
+        for (int w = 0; w < NW; w++) nd[w] = bswap32(node[w]);
+        const uint32_t* ap = auth + h * NW;
+        thash_sha2_pair<NW, 2 * NW>(mid, ha, [&](int j) {
+            int i = j + half;
+            i = i < 0 ? 0 : i < 2 * NW ? i : 2 * NW - 1;
+            const uint32_t* src = (i < NW) == right ? ap : nd;
+            return bswap32(src[i < NW ? i : i - NW]);
+        }, node);
+    }
+}
+
 // verify: the d layers of R = 256 / G requests per workgroup in order
 // (sign.cu:3077-3113), so G is the mean number of lanes per request.  Per
 // layer the R * len WOTS chains of the workgroup (15 - len_j steps each,
@@ -940,6 +1033,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     __shared__ uint32_t sig_s[R][TLW];  // WOTS signature of the layer, raw words
     __shared__ uint32_t pk_s[R][TLW];   // chain ends (SHAKE256: interleaved)
     __shared__ uint32_t auth_s[R][AW];  // auth path of the layer, raw words
+    __shared__ uint32_t node_s[SHA2 ? R : 1][NW];  // climb: the node, raw words
     __shared__ uint16_t order_s[NCH];   // rank -> chain id | remaining steps << 12
     __shared__ uint32_t hist_s[16];     // chains per remaining-step count
 
@@ -1090,44 +1184,53 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
             if (more && u < nact * TLW) sig_s[u / TLW][u % TLW] = nx[k];
         }
         if (tid < 16) hist_s[tid] = 0;
-        if (tid < nact) {  // T_len and the subtree climb of request tid
-            const uint64_t tree = tree_s[tid];
-            const uint32_t leaf = leaf_s[tid];
+        // T_len and the subtree climb of request rq: SHA-256 on the lane pair
+        // 2 rq, 2 rq + 1 (sha256_compress_pair), SHAKE256 on lane rq
+        const int rq = SHA2 ? tid >> 1 : tid;
+        if (tid < (SHA2 ? 2 * nact : nact)) {
+            const uint64_t tree = tree_s[rq];
+            const uint32_t leaf = leaf_s[rq];
             Key k2;
 #pragma unroll
             for (int i = 0; i < NW; i++) k2.sk[i] = 0;
             if constexpr (SHA2) {
 #pragma unroll
-                for (int i = 0; i < 8; i++) k2.mid[i] = key_s[tid][i];
+                for (int i = 0; i < 8; i++) k2.mid[i] = key_s[rq][i];
             } else {
 #pragma unroll
-                for (int i = 0; i < NW; i++) k2.pub[i] = key_s[tid][i];
+                for (int i = 0; i < NW; i++) k2.pub[i] = key_s[rq][i];
             }
-            Adr pa = adr_make(layer, tree, kWotsPk);
-            adr_keypair<TH>(pa, leaf);
             uint32_t node[NW];
-            if constexpr (SHA2) tlen_lds<P>(k2.mid, pa, &pk_s[tid][0], node);
-            else tlen_lds_il<P>(k2, pa, &pk_s[tid][0], node);
-            uint32_t li = leaf;
-            Adr ha = adr_make(layer, tree, kHashTree);
+            if constexpr (SHA2) {
+                tlen_climb_pair<P>(k2.mid, layer, tree, leaf, &pk_s[rq][0], &auth_s[rq][0],
+                                   &node_s[rq][0], node);
+            } else {
+                Adr pa = adr_make(layer, tree, kWotsPk);
+                adr_keypair<TH>(pa, leaf);
+                tlen_lds_il<P>(k2, pa, &pk_s[rq][0], node);
+                uint32_t li = leaf;
+                Adr ha = adr_make(layer, tree, kHashTree);
 #pragma unroll 1
-            for (int h = 0; h < TH; h++) {
-                uint32_t lft[NW], rgt[NW];
-                const bool odd = li & 1u;
-                li >>= 1;
-                adr_node(ha, h + 1, li);
+                for (int h = 0; h < TH; h++) {
+                    uint32_t lft[NW], rgt[NW];
+                    const bool right = li & 1u;
+                    li >>= 1;
+                    adr_node(ha, h + 1, li);
 #pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    const uint32_t ap = to_native<P::hash>(auth_s[tid][h * NW + w]);
-                    lft[w] = odd ? ap : node[w];
-                    rgt[w] = odd ? node[w] : ap;
+                    for (int w = 0; w < NW; w++) {
+                        const uint32_t apw = to_native<P::hash>(auth_s[rq][h * NW + w]);
+                        lft[w] = right ? apw : node[w];
+                        rgt[w] = right ? node[w] : apw;
+                    }
+                    Hh::H(k2, ha, lft, rgt, node);
                 }
-                Hh::H(k2, ha, lft, rgt, node);
             }
+            if (!SHA2 || (tid & 1) == 0) {
 #pragma unroll
-            for (int w = 0; w < NW; w++) root_s[tid][w] = node[w];
-            leaf_s[tid] = (uint32_t)(tree & ((1u << TH) - 1));
-            tree_s[tid] = tree >> TH;
+                for (int w = 0; w < NW; w++) root_s[rq][w] = node[w];
+                leaf_s[rq] = (uint32_t)(tree & ((1u << TH) - 1));
+                tree_s[rq] = tree >> TH;
+            }
         }
     }
     __syncthreads();
@@ -1231,6 +1334,7 @@ __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
     __shared__ uint32_t pk_s[TLW];    // chain ends (SHAKE256: interleaved)
     __shared__ uint32_t auth_s[AW];   // auth path, raw words
     __shared__ uint32_t root_s[NW];
+    __shared__ uint32_t node_s[NW];   // climb (SHA-256 lane pair): the node, raw words
     const int tid = threadIdx.x;
     const uint32_t req = blockIdx.x;  // grid = n_req
     const Plan& pl = A.plan[req];
@@ -1261,14 +1365,24 @@ __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
             sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes + P::wots_bytes);
         for (int u = tid; u < AW; u += 64) auth_s[u] = auth[u];
         __syncthreads();
-        if (tid == 0) {
+        if constexpr (SHA2) {  // lanes 0 and 1
+            if (tid < 2) {
+                Key key;
+                Hh::load_key(pl, key);
+                uint32_t node[NW];
+                tlen_climb_pair<P>(key.mid, layer, tree, leaf, pk_s, auth_s, node_s, node);
+                if (tid == 0) {
+#pragma unroll
+                    for (int w = 0; w < NW; w++) root_s[w] = node[w];
+                }
+            }
+        } else if (tid == 0) {
             Key key;
             Hh::load_key(pl, key);
             Adr pa = adr_make(layer, tree, kWotsPk);
             adr_keypair<TH>(pa, leaf);
             uint32_t node[NW];
-            if constexpr (SHA2) tlen_lds<P>(key.mid, pa, pk_s, node);
-            else tlen_lds_il<P>(key, pa, pk_s, node);
+            tlen_lds_il<P>(key, pa, pk_s, node);
             uint32_t li = leaf;
             Adr ha = adr_make(layer, tree, kHashTree);
 #pragma unroll 1

@@ -281,3 +281,31 @@ def test_large_batch_launch_paths(name, num, S):
         assert bytes(pk[i]) == opk and bytes(sk[i]) == osk, i
         assert bytes(sig[i]) == O.signature_with_optrand(pid, msgs[i], osk, opts[i]), i
     assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * num
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,S", [("sha2-128f", 2), ("shake256-192f", 1), ("sha2-128s", 2)])
+def test_fors_pk_inside_and_outside_the_fors_tiles(name, S):
+    """Sign batches of up to 256 requests compute each FORS public key on the
+    lane that stores the request's last FORS root; larger batches launch a
+    separate T_k kernel.  The same 257 requests signed as one batch (separate
+    T_k) and as batches of 256 and 1 (T_k in the tiles; 128s at S = 2 runs
+    the separate FORS and hypertree kernels) must give identical bytes, and
+    a sample must match the oracle."""
+    pid = C.param_id(name)
+    num = 257
+    seeds, msgs, opts = _inputs(pid, num, 9090 + pid, [32, 5, 300])
+    opt = np.frombuffer(b"".join(opts), np.uint8)
+    try:
+        C.set_split(pid, C.OP_SIGN, S)
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+        whole = C.batch_sign(pid, sk, msgs, opt)
+        p = O.params(pid)
+        head = C.batch_sign(pid, sk[:256], msgs[:256], opt[:256 * p.n])
+        tail = C.batch_sign(pid, sk[256:], msgs[256:], opt[256 * p.n:])
+    finally:
+        C.set_split(pid, C.OP_SIGN, 0)
+    assert np.array_equal(whole[:256], head)
+    assert np.array_equal(whole[256:], tail)
+    for i in (0, 255, 256):
+        assert bytes(whole[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), i
