@@ -140,6 +140,32 @@ __device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32
     s[3] += q3;
 }
 
+// sha256_compress on a lane pair whose lanes both hold the whole chaining
+// value st and block w (the message-hash paths): the halves are split for
+// sha256_compress_pair and st is whole again in both lanes afterwards
+__device__ __forceinline__ void sha256_compress_dup(uint32_t st[8], const uint32_t w[16]) {
+    const uint32_t m = (threadIdx.x & 1u) ? ~0u : 0u;
+    uint32_t s4[4], v[8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) s4[i] = lane_sel(m, st[i], st[4 + i]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = lane_sel(m, w[2 * k + 1], w[2 * k]);
+    sha256_compress_pair(s4, v);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t other = pair_swap(s4[i]);
+        st[i] = lane_sel(m, s4[i], other);
+        st[4 + i] = lane_sel(m, other, s4[i]);
+    }
+}
+
+// one compression: on one lane, or on a lane pair holding identical inputs
+template <bool PAIR>
+__device__ __forceinline__ void sha256_compress_by(uint32_t st[8], uint32_t w[16]) {
+    if constexpr (PAIR) sha256_compress_dup(st, w);
+    else sha256_compress(st, w);
+}
+
 // write x into b[q] for a wave-uniform runtime q (v_cndmask per slot)
 __device__ __forceinline__ void put16(uint32_t b[16], int q, uint32_t x) {
 #pragma unroll
@@ -291,13 +317,15 @@ struct Sha2 {
     // ------------------------------------------------ per-request hashing
 
     // midstate: compress(IV, pub_seed || 0^(64-n)) (seed_state, sha256.cu:391)
+    // PAIR (here and below): run by a lane pair holding identical inputs
+    template <bool PAIR = false>
     __device__ static __forceinline__ void midstate(const uint32_t pub[NW], uint32_t mid[8]) {
         uint32_t w[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) w[i] = i < NW ? pub[i] : 0u;
 #pragma unroll
         for (int i = 0; i < 8; i++) mid[i] = kSha256IV[i];
-        sha256_compress(mid, w);
+        sha256_compress_by<PAIR>(mid, w);
     }
 
     // SHA-256 continuing from st over (pre[0..NPRE) || m[0..mlen)), where
@@ -305,7 +333,7 @@ struct Sha2 {
     // mlen are never read beyond the word holding the last byte.  Blocks made
     // only of whole message words are loaded directly; the head (prefix) and
     // the tail (last partial word, padding, length) go word by word.
-    template <int NPRE>
+    template <int NPRE, bool PAIR = false>
     __device__ static void stream_msg(uint32_t st[8], const uint32_t pre[NPRE],
                                       const uint8_t* m, uint32_t mlen, uint32_t before) {
         const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
@@ -323,7 +351,7 @@ struct Sha2 {
                 const uint32_t* src = mw + (kk - NPRE);
 #pragma unroll
                 for (int i = 0; i < 16; i++) b[i] = bswap32(src[i]);
-                sha256_compress(st, b);
+                sha256_compress_by<PAIR>(st, b);
                 kk += 16;
                 continue;
             }
@@ -350,12 +378,13 @@ struct Sha2 {
                 x = 0;
             }
             put16(b, (int)(kk & 15), x);
-            if ((kk & 15) == 15) sha256_compress(st, b);
+            if ((kk & 15) == 15) sha256_compress_by<PAIR>(st, b);
             kk++;
         }
     }
 
     // R = HMAC-SHA256(sk_prf, optrand || m)[:n]  (hash_sha256.cu:99-144)
+    template <bool PAIR = false>
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
                                    const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
         uint32_t st[8], w[16], inner[8];
@@ -363,25 +392,25 @@ struct Sha2 {
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
 #pragma unroll
         for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x36363636u;
-        sha256_compress(st, w);
-        stream_msg<NW>(st, opt, m, mlen, 64);
+        sha256_compress_by<PAIR>(st, w);
+        stream_msg<NW, PAIR>(st, opt, m, mlen, 64);
 #pragma unroll
         for (int i = 0; i < 8; i++) inner[i] = st[i];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
 #pragma unroll
         for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x5c5c5c5cu;
-        sha256_compress(st, w);
+        sha256_compress_by<PAIR>(st, w);
 #pragma unroll
         for (int i = 0; i < 16; i++)
             w[i] = i < 8 ? inner[i] : (i == 8 ? 0x80000000u : (i == 15 ? 768u : 0u));
-        sha256_compress(st, w);
+        sha256_compress_by<PAIR>(st, w);
 #pragma unroll
         for (int i = 0; i < NW; i++) R[i] = st[i];
     }
 
     // digest bytes = MGF1(SHA256(R || pk || m), DG)  (hash_sha256.cu:200-247)
-    template <int DG>
+    template <int DG, bool PAIR = false>
     __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
                                  const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
         uint32_t pre[3 * NW], st[8], seed[8], w[16];
@@ -391,7 +420,7 @@ struct Sha2 {
         for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
-        stream_msg<3 * NW>(st, pre, m, mlen, 0);
+        stream_msg<3 * NW, PAIR>(st, pre, m, mlen, 0);
 #pragma unroll
         for (int i = 0; i < 8; i++) seed[i] = st[i];
 #pragma unroll
@@ -403,7 +432,7 @@ struct Sha2 {
                                        : (i == 9 ? 0x80000000u : (i == 15 ? 288u : 0u)));
 #pragma unroll
             for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
-            sha256_compress(st, w);
+            sha256_compress_by<PAIR>(st, w);
 #pragma unroll
             for (int i = 0; i < 32; i++)
                 if (32 * c + i < DG) dg[32 * c + i] = (uint8_t)(st[i >> 2] >> (24 - 8 * (i & 3)));

@@ -436,6 +436,9 @@ struct Shake {
         for (int o = 0; o < OUTB; o++) out[o] = (uint8_t)(s[o >> 2] >> (8 * (o & 3)));
     }
 
+    // (PAIR: the SHA-256 lane-pair form; SHAKE256 requests always hash on
+    // one lane, so it is false here)
+    template <bool PAIR = false>
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
                                    const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
         uint32_t pre[2 * NW];
@@ -452,7 +455,7 @@ struct Shake {
                    (uint32_t)o[4 * i + 2] << 16 | (uint32_t)o[4 * i + 3] << 24;
     }
 
-    template <int DG>
+    template <int DG, bool PAIR = false>
     __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
                                  const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
         uint32_t pre[3 * NW];
@@ -463,6 +466,7 @@ struct Shake {
         stream_msg<3 * NW, DG>(pre, m, mlen, dg);
     }
 
+    template <bool PAIR = false>
     __device__ static __forceinline__ void midstate(const uint32_t*, uint32_t mid[8]) {
 #pragma unroll
         for (int i = 0; i < 8; i++) mid[i] = 0;
