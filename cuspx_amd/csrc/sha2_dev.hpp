@@ -83,18 +83,6 @@ __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) 
 // per schedule pair against 14 and 20 on one lane: for serial hashes whose
 // lane would otherwise run alone (T_len and Merkle climbs in verification).
 
-// the other lane of the pair (DPP quad_perm [1, 0, 3, 2])
-__device__ __forceinline__ uint32_t pair_swap(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
-}
-
-// m ? a : b bitwise, for a lane mask m.  Written as a bit select so that a
-// choice between two elements of one array never becomes an indexed read
-// (which the compiler lowers to a compare-and-select chain over the array).
-__device__ __forceinline__ uint32_t lane_sel(uint32_t m, uint32_t a, uint32_t b) {
-    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
-}
-
 __device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32_t vin[8]) {
     const bool odd = threadIdx.x & 1u;
     const uint32_t m = odd ? ~0u : 0u;

@@ -273,6 +273,64 @@ __device__ __forceinline__ void keccak_f_il(uint32_t s[50]) {
         keccak_round_il(s, kKeccakRcIl.w[2 * r], kKeccakRcIl.w[2 * r + 1]);
 }
 
+// Keccak-f[1600] on a lane pair in the interleaved form: the even lane holds
+// the E halves of the 25 lanes, the odd lane the O halves (s[i] = this
+// lane's half of Keccak lane i).  A rotation by 2k rotates both halves by k
+// (no exchange); a rotation by 2k + 1 swaps them (one DPP move) and rotates
+// by k + 1 (even lane) or k (odd lane), a per-lane count.  Theta's column
+// sums, chi and iota stay within a half.  About 108 instructions per round
+// on each lane against 174 for the whole state on one lane: for serial
+// sponges whose lane would otherwise run alone.
+__device__ __forceinline__ void keccak_round_il_pair(uint32_t s[25], uint32_t m, uint32_t rc) {
+    uint32_t C[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) C[x] = xor3(xor3(s[x], s[x + 5], s[x + 10]), s[x + 15], s[x + 20]);
+    const uint32_t sh1 = lane_sel(m, 0u, 31u);  // rotl by 1 (E) or 0 (O) after the swap
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        const uint32_t t = pair_swap(C[(x + 1) % 5]);
+        const uint32_t r = __builtin_amdgcn_alignbit(t, t, sh1);  // rol64(C[x+1], 1), this half
+#pragma unroll
+        for (int y = 0; y < 5; y++) s[x + 5 * y] = xor3(s[x + 5 * y], C[(x + 4) % 5], r);
+    }
+    uint32_t B[25];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            const int src = x + 5 * y;
+            const int dst = y + 5 * ((2 * x + 3 * y) % 5);
+            const int R = kRho[src];
+            uint32_t v;
+            if (R == 0) {
+                v = s[src];
+            } else if ((R & 1) == 0) {
+                v = __builtin_amdgcn_alignbit(s[src], s[src], (32 - R / 2) & 31);
+            } else {
+                const uint32_t t = pair_swap(s[src]);
+                const uint32_t sh = lane_sel(m, (uint32_t)((32 - R / 2) & 31),
+                                             (uint32_t)((31 - R / 2) & 31));
+                v = __builtin_amdgcn_alignbit(t, t, sh);
+            }
+            B[dst] = v;
+        }
+    }
+#pragma unroll
+    for (int y = 0; y < 5; y++)
+#pragma unroll
+        for (int x = 0; x < 5; x++)
+            s[x + 5 * y] = bit_chi(B[x + 5 * y], B[(x + 1) % 5 + 5 * y], B[(x + 2) % 5 + 5 * y]);
+    s[0] ^= rc;
+}
+
+__device__ __forceinline__ void keccak_f_il_pair(uint32_t s[25]) {
+    const uint32_t m = (threadIdx.x & 1u) ? ~0u : 0u;
+    keccak_round_il_pair(s, m, lane_sel(m, 0u, 1u));  // RC[0] = 1: bit 0 is even
+#pragma unroll 1
+    for (int r = 1; r < 24; r++)
+        keccak_round_il_pair(s, m, lane_sel(m, kKeccakRcIl.w[2 * r + 1], kKeccakRcIl.w[2 * r]));
+}
+
 // xor x into s[q] for a wave-uniform runtime q < 34 (the rate)
 __device__ __forceinline__ void xor_at(uint32_t s[50], int q, uint32_t x) {
 #pragma unroll

@@ -84,6 +84,19 @@ __device__ __forceinline__ uint32_t bit_chi(uint32_t a, uint32_t b, uint32_t c) 
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);  // a ^ (~b & c)
 }
 
+// Lane pairs (lanes 2i, 2i + 1 of a wave, both active) share one serial
+// hash: the other lane's value of x (DPP quad_perm [1, 0, 3, 2])
+__device__ __forceinline__ uint32_t pair_swap(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
+}
+
+// m ? a : b bitwise, for a lane mask m.  Written as a bit select so that a
+// choice between two elements of one array never becomes an indexed read
+// (which the compiler lowers to a compare-and-select chain over the array).
+__device__ __forceinline__ uint32_t lane_sel(uint32_t m, uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
+}
+
 // ------------------------------------------------------------- addresses
 //
 // The 32-byte address as 8 big-endian words: word i holds bytes 4i..4i+3.

@@ -569,14 +569,80 @@ __device__ __forceinline__ void thash_shake_rd(const typename Shake<P::n>::Key& 
     }
 }
 
-// T_len of one WOTS public key held in LDS as bit-interleaved words (SHAKE256)
-template <class P>
-__device__ __forceinline__ void tlen_lds_il(const typename Shake<P::n>::Key& key, const Adr& a,
-                                            const uint32_t* pk, uint32_t out[P::nw]) {
-    thash_shake_rd<P, P::len * P::nw, true>(key, a, [&](int i) { return pk[i]; }, out);
+// SHAKE256 thash on a lane pair in the interleaved domain (both lanes
+// active, same key, address and payload; keccak_f_il_pair): rd2(j), j even,
+// returns interleaved payload word j + (lane & 1), clamped to 0..M-1, i.e.
+// this lane's half of payload lane j / 2.  The output (plain) is in both
+// lanes.
+template <class P, int M, class Rd2>
+__device__ __forceinline__ void thash_shake_pair(const typename Shake<P::n>::Key& key,
+                                                 const Adr& a, const Rd2& rd2,
+                                                 uint32_t out[P::nw]) {
+    using Hh = Shake<P::n>;
+    constexpr int NW = P::nw;
+    constexpr int PRE = NW + 8;                // pub_seed || address, interleaved words
+    constexpr int T = PRE + M;                 // absorbed words
+    constexpr int NB = T / 34 + 1;             // permutations (the pad always fits)
+    constexpr int TAIL = T - 34 * (NB - 1);    // words in the last block
+    static_assert((PRE & 1) == 0, "the prefix fills whole lanes");
+    const bool odd = threadIdx.x & 1u;
+    const uint32_t m = odd ? ~0u : 0u;
+    typename Hh::ChainIl c;
+    Hh::chain_il_begin(key, a, c);
+    uint32_t s[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) s[i] = 0;
+    // interleaved word w of the last block: the 0x1F pad (byte 4 TAIL) and
+    // 0x80 (byte 135) in their interleaved bits
+    auto pad = [](int w) -> uint32_t {
+        uint32_t v = 0;
+        if ((TAIL & 1) == 0) v ^= w == TAIL ? 7u : w == TAIL + 1 ? 3u : 0u;
+        else v ^= w == TAIL - 1 ? 7u << 16 : w == TAIL ? 3u << 16 : 0u;
+        return v ^ (w == 33 ? 0x80000000u : 0u);
+    };
+    // block tb (a constant once unrolled): stream word g = 34 tb + 2 L + half
+    // is prefix, payload (this lane's half from rd2), zero, and the pad in
+    // the last block
+    auto block = [&](int tb) {
+#pragma unroll
+        for (int L = 0; L < 17; L++) {
+            const int g0 = 34 * tb + 2 * L;
+            const uint32_t x = (g0 >= PRE && g0 < T) ? rd2(g0 - PRE) : 0u;
+            uint32_t y[2];
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int g = g0 + p;
+                uint32_t v = g < PRE ? c.w[g < PRE ? g : 0] : g < T ? x : 0u;
+                if (tb == NB - 1) v ^= pad(g - 34 * (NB - 1));
+                y[p] = v;
+            }
+            s[L] ^= lane_sel(m, y[1], y[0]);
+        }
+    };
+#pragma unroll 1
+    for (int blk = 0; blk < NB; blk++) {
+        if (blk == 0) {
+            block(0);
+        } else if (blk < NB - 1) {  // payload only
+            const int base = 34 * blk - PRE;
+#pragma unroll
+            for (int L = 0; L < 17; L++) s[L] ^= rd2(base + 2 * L);
+        } else {
+            block(NB - 1);
+        }
+        keccak_f_il_pair(s);
+    }
+    uint32_t il[NW];
+#pragma unroll
+    for (int L = 0; L < NW / 2; L++) {
+        const uint32_t other = pair_swap(s[L]);
+        il[2 * L] = lane_sel(m, other, s[L]);
+        il[2 * L + 1] = lane_sel(m, s[L], other);
+    }
+    Hh::from_il(il, out);
 }
 
-// FORS public key of request i: T_k over its k roots (fors.cu:142); the
+
 // roots are read past the caches, as another workgroup may have stored them
 template <class P>
 __device__ __forceinline__ void fors_pk_of(const BatchArgs& A, uint32_t i) {
@@ -980,25 +1046,28 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     }
 }
 
-// Verification of one layer's subtree, SHA-256 on a lane pair (both lanes
-// active): T_len over the WOTS public key in LDS (native words, pk) and the
-// climb (compute_root, utils.cu:94-135) with the auth path in LDS (raw
-// words, auth); nd is NW words of LDS for the node of the climb.  node is
-// the layer root, in both lanes.
+// Verification of one layer's subtree on a lane pair (both lanes active):
+// T_len over the WOTS public key in LDS (pk: native words for SHA-256,
+// interleaved for SHAKE256) and the climb (compute_root, utils.cu:94-135)
+// with the auth path in LDS (raw words); nd is NW words of LDS for the node
+// of the climb.  node is the layer root, in both lanes.
 template <class P>
-__device__ __forceinline__ void tlen_climb_pair(const uint32_t mid[8], uint32_t layer,
-                                                uint64_t tree, uint32_t leaf, const uint32_t* pk,
-                                                const uint32_t* auth, uint32_t* nd,
-                                                uint32_t node[P::nw]) {
+__device__ __forceinline__ void tlen_climb_pair(const typename HashT<P>::Key& key,
+                                                uint32_t layer, uint64_t tree, uint32_t leaf,
+                                                const uint32_t* pk, const uint32_t* auth,
+                                                uint32_t* nd, uint32_t node[P::nw]) {
     constexpr int NW = P::nw;
     constexpr int TLW = P::len * NW;
+    constexpr bool SHA2 = P::hash == kSha2;
     const int half = threadIdx.x & 1;
     Adr pa = adr_make(layer, tree, kWotsPk);
     adr_keypair<P::th>(pa, leaf);
-    thash_sha2_pair<NW, TLW>(mid, pa, [&](int j) {
+    auto rpk = [&](int j) {
         const int i = j + half;
         return pk[i < 0 ? 0 : i < TLW ? i : TLW - 1];
-    }, node);
+    };
+    if constexpr (SHA2) thash_sha2_pair<NW, TLW>(key.mid, pa, rpk, node);
+    else thash_shake_pair<P, TLW>(key, pa, rpk, node);
     uint32_t li = leaf;
     Adr ha = adr_make(layer, tree, kHashTree);
 #pragma unroll 1
@@ -1006,16 +1075,31 @@ __device__ __forceinline__ void tlen_climb_pair(const uint32_t mid[8], uint32_t 
         const bool right = li & 1u;  // node is the right child
         li >>= 1;
         adr_node(ha, h + 1, li);
-        // payload node || auth or auth || node, raw words from LDS
+        // payload node || auth or auth || node from LDS (nd: the node as raw
+        // words, like the auth path)
 #pragma unroll
-        for (int w = 0; w < NW; w++) nd[w] = bswap32(node[w]);
+        for (int w = 0; w < NW; w++) nd[w] = SHA2 ? bswap32(node[w]) : node[w];
         const uint32_t* ap = auth + h * NW;
-        thash_sha2_pair<NW, 2 * NW>(mid, ha, [&](int j) {
-            int i = j + half;
-            i = i < 0 ? 0 : i < 2 * NW ? i : 2 * NW - 1;
+        auto word = [&](int i) {  // payload word i (raw)
             const uint32_t* src = (i < NW) == right ? ap : nd;
-            return bswap32(src[i < NW ? i : i - NW]);
-        }, node);
+            return src[i < NW ? i : i - NW];
+        };
+        if constexpr (SHA2) {
+            thash_sha2_pair<NW, 2 * NW>(key.mid, ha, [&](int j) {
+                int i = j + half;
+                i = i < 0 ? 0 : i < 2 * NW ? i : 2 * NW - 1;
+                return bswap32(word(i));
+            }, node);
+        } else {
+            // this lane's interleaved half of payload lane j / 2
+            thash_shake_pair<P, 2 * NW>(key, ha, [&](int j) {
+                int l = j >> 1;
+                l = l < 0 ? 0 : l < NW ? l : NW - 1;
+                uint32_t e, o;
+                il_to(word(2 * l), word(2 * l + 1), e, o);
+                return half ? o : e;
+            }, node);
+        }
     }
 }
 
@@ -1055,7 +1139,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     __shared__ uint32_t sig_s[R][TLW];  // WOTS signature of the layer, raw words
     __shared__ uint32_t pk_s[R][TLW];   // chain ends (SHAKE256: interleaved)
     __shared__ uint32_t auth_s[R][AW];  // auth path of the layer, raw words
-    __shared__ uint32_t node_s[SHA2 ? R : 1][NW];  // climb: the node, raw words
+    __shared__ uint32_t node_s[R][NW];  // climb: the node, raw words
     __shared__ uint16_t order_s[NCH];   // rank -> chain id | remaining steps << 12
     __shared__ uint32_t hist_s[16];     // chains per remaining-step count
 
@@ -1206,10 +1290,10 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
             if (more && u < nact * TLW) sig_s[u / TLW][u % TLW] = nx[k];
         }
         if (tid < 16) hist_s[tid] = 0;
-        // T_len and the subtree climb of request rq: SHA-256 on the lane pair
-        // 2 rq, 2 rq + 1 (sha256_compress_pair), SHAKE256 on lane rq
-        const int rq = SHA2 ? tid >> 1 : tid;
-        if (tid < (SHA2 ? 2 * nact : nact)) {
+        // T_len and the subtree climb of request rq on the lane pair 2 rq,
+        // 2 rq + 1 (sha256_compress_pair / keccak_f_il_pair)
+        const int rq = tid >> 1;
+        if (tid < 2 * nact) {
             const uint64_t tree = tree_s[rq];
             const uint32_t leaf = leaf_s[rq];
             Key k2;
@@ -1223,31 +1307,9 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
                 for (int i = 0; i < NW; i++) k2.pub[i] = key_s[rq][i];
             }
             uint32_t node[NW];
-            if constexpr (SHA2) {
-                tlen_climb_pair<P>(k2.mid, layer, tree, leaf, &pk_s[rq][0], &auth_s[rq][0],
-                                   &node_s[rq][0], node);
-            } else {
-                Adr pa = adr_make(layer, tree, kWotsPk);
-                adr_keypair<TH>(pa, leaf);
-                tlen_lds_il<P>(k2, pa, &pk_s[rq][0], node);
-                uint32_t li = leaf;
-                Adr ha = adr_make(layer, tree, kHashTree);
-#pragma unroll 1
-                for (int h = 0; h < TH; h++) {
-                    uint32_t lft[NW], rgt[NW];
-                    const bool right = li & 1u;
-                    li >>= 1;
-                    adr_node(ha, h + 1, li);
-#pragma unroll
-                    for (int w = 0; w < NW; w++) {
-                        const uint32_t apw = to_native<P::hash>(auth_s[rq][h * NW + w]);
-                        lft[w] = right ? apw : node[w];
-                        rgt[w] = right ? node[w] : apw;
-                    }
-                    Hh::H(k2, ha, lft, rgt, node);
-                }
-            }
-            if (!SHA2 || (tid & 1) == 0) {
+            tlen_climb_pair<P>(k2, layer, tree, leaf, &pk_s[rq][0], &auth_s[rq][0], &node_s[rq][0],
+                               node);
+            if ((tid & 1) == 0) {
 #pragma unroll
                 for (int w = 0; w < NW; w++) root_s[rq][w] = node[w];
                 leaf_s[rq] = (uint32_t)(tree & ((1u << TH) - 1));
@@ -1356,7 +1418,7 @@ __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
     __shared__ uint32_t pk_s[TLW];    // chain ends (SHAKE256: interleaved)
     __shared__ uint32_t auth_s[AW];   // auth path, raw words
     __shared__ uint32_t root_s[NW];
-    __shared__ uint32_t node_s[NW];   // climb (SHA-256 lane pair): the node, raw words
+    __shared__ uint32_t node_s[NW];   // climb: the node, raw words
     const int tid = threadIdx.x;
     const uint32_t req = blockIdx.x;  // grid = n_req
     const Plan& pl = A.plan[req];
@@ -1387,42 +1449,15 @@ __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
             sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes + P::wots_bytes);
         for (int u = tid; u < AW; u += 64) auth_s[u] = auth[u];
         __syncthreads();
-        if constexpr (SHA2) {  // lanes 0 and 1
-            if (tid < 2) {
-                Key key;
-                Hh::load_key(pl, key);
-                uint32_t node[NW];
-                tlen_climb_pair<P>(key.mid, layer, tree, leaf, pk_s, auth_s, node_s, node);
-                if (tid == 0) {
-#pragma unroll
-                    for (int w = 0; w < NW; w++) root_s[w] = node[w];
-                }
-            }
-        } else if (tid == 0) {
+        if (tid < 2) {  // T_len and the climb on the lane pair 0, 1
             Key key;
             Hh::load_key(pl, key);
-            Adr pa = adr_make(layer, tree, kWotsPk);
-            adr_keypair<TH>(pa, leaf);
             uint32_t node[NW];
-            tlen_lds_il<P>(key, pa, pk_s, node);
-            uint32_t li = leaf;
-            Adr ha = adr_make(layer, tree, kHashTree);
-#pragma unroll 1
-            for (int h = 0; h < TH; h++) {
-                uint32_t lft[NW], rgt[NW];
-                const bool odd = li & 1u;
-                li >>= 1;
-                adr_node(ha, h + 1, li);
+            tlen_climb_pair<P>(key, layer, tree, leaf, pk_s, auth_s, node_s, node);
+            if (tid == 0) {
 #pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    const uint32_t ap = to_native<P::hash>(auth_s[h * NW + w]);
-                    lft[w] = odd ? ap : node[w];
-                    rgt[w] = odd ? node[w] : ap;
-                }
-                Hh::H(key, ha, lft, rgt, node);
+                for (int w = 0; w < NW; w++) root_s[w] = node[w];
             }
-#pragma unroll
-            for (int w = 0; w < NW; w++) root_s[w] = node[w];
         }
         leaf = (uint32_t)(tree & ((1u << TH) - 1));
         tree >>= TH;
