@@ -791,8 +791,8 @@ __device__ __forceinline__ void ht_node(const BatchArgs& A, uint32_t g, uint32_t
 // the most work per lane (throughput), larger S shortens the per-lane chain
 // of dependent compressions (latency of small batches); the launcher picks S.
 // With S > 1 (and always for SHAKE) the chain outputs are parked in tl_buf
-// ([word][leaf], coalesced) and absorbed into T_len by lane q = 0 (SHAKE256)
-// or the lane pair q = 0, 1 (SHA-256, sha256_compress_pair).
+// ([word][leaf], coalesced) and absorbed into T_len by the lane pair q = 0, 1
+// (sha256_compress_pair / keccak_f_il_pair; SHAKE256 at S = 1: lane q = 0).
 //
 // A subtree whose 2^h' x S lanes exceed one 1024-lane workgroup (the s sets
 // at S >= 4) is split over PARTS workgroups of 256 lanes.  Each part reduces
@@ -900,20 +900,21 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     }
     if constexpr (S > 1) __syncthreads();  // all chains of the leaf parked
     uint32_t node[NW];
-    // T_len: SHA-256 with parked chain ends on the lane pair q = 0, 1 of the
-    // leaf (S is even, so the pair is lanes 2i, 2i + 1), else on lane q = 0
-    constexpr bool pair = P::hash == kSha2 && Sh::park;
+    // T_len: with S > 1 on the lane pair q = 0, 1 of the leaf (S is even, so
+    // the pair is lanes 2i, 2i + 1), else on lane q = 0
+    constexpr bool pair = S > 1;
     if (q < (pair ? 2 : 1)) {
-        if constexpr (pair) {
-            constexpr int TLW = P::len * NW;
-            thash_sha2_pair<NW, TLW>(key.mid, pa, [&](int j) {
-                const int i = j + q;
-                return parked(i < 0 ? 0 : i < TLW ? i : TLW - 1);
-            }, node);
-        } else if constexpr (P::hash == kSha2) {
-            ts.template finish<P::len * NW>(node);
+        constexpr int TLW = P::len * NW;
+        auto parked2 = [&](int j) {
+            const int i = j + q;
+            return parked(i < 0 ? 0 : i < TLW ? i : TLW - 1);
+        };
+        if constexpr (P::hash == kSha2) {
+            if constexpr (pair) thash_sha2_pair<NW, TLW>(key.mid, pa, parked2, node);
+            else ts.template finish<TLW>(node);
         } else {
-            thash_shake_rd<P, P::len * NW, true>(key, pa, parked, node);
+            if constexpr (pair) thash_shake_pair<P, TLW>(key, pa, parked2, node);
+            else thash_shake_rd<P, TLW, true>(key, pa, parked, node);
         }
         if (q == 0) {
             uint8_t* auth = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
