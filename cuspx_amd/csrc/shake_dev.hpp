@@ -494,8 +494,68 @@ struct Shake {
         for (int o = 0; o < OUTB; o++) out[o] = (uint8_t)(s[o >> 2] >> (8 * (o & 3)));
     }
 
-    // (PAIR: the SHA-256 lane-pair form; SHAKE256 requests always hash on
-    // one lane, so it is false here)
+    // stream_msg on a lane pair (both lanes active with identical inputs;
+    // keccak_f_il_pair): the state is held interleaved, this lane's half of
+    // every Keccak lane, and each absorbed plain word is split into its even
+    // or odd bits on the way in.  The output is in both lanes.
+    template <int NPRE, int OUTB>
+    __device__ static void stream_msg_pair(const uint32_t pre[NPRE], const uint8_t* m,
+                                           uint32_t mlen, uint8_t out[OUTB]) {
+        static_assert(NPRE < 34 && OUTB <= 136, "prefix within one block");
+        const uint32_t h = threadIdx.x & 1u;
+        const uint32_t msk = h ? ~0u : 0u;
+        // this lane's interleaved half of plain word x at stream word q
+        auto half = [&](uint32_t x, int q) { return il_even16(x >> h) << (16 * (q & 1)); };
+        const uint32_t* mw = reinterpret_cast<const uint32_t*>(m);
+        uint32_t s[25];
+#pragma unroll
+        for (int i = 0; i < 25; i++) s[i] = 0;
+#pragma unroll
+        for (int q = 0; q < NPRE; q++) s[q >> 1] ^= half(pre[q], q);
+        int q = NPRE;
+        const uint32_t full = mlen / 4, rem = mlen & 3;
+        uint32_t i = 0;
+        // xor this lane's half of word x at a wave-uniform runtime q
+        auto put = [&](int qq, uint32_t x) {
+            const uint32_t c = il_even16(x >> h) << (16 * (qq & 1));
+#pragma unroll
+            for (int l = 0; l < 17; l++) s[l] ^= (qq >> 1) == l ? c : 0u;
+        };
+#pragma unroll 1
+        while (i < full) {
+            if (q == 0 && i + 34 <= full) {
+#pragma unroll
+                for (int l = 0; l < 17; l++)
+                    s[l] ^= il_even16(mw[i + 2 * l] >> h) | (il_even16(mw[i + 2 * l + 1] >> h) << 16);
+                keccak_f_il_pair(s);
+                i += 34;
+                continue;
+            }
+            put(q, mw[i]);
+            q++;
+            i++;
+            if (q == 34) {
+                keccak_f_il_pair(s);
+                q = 0;
+            }
+        }
+        uint32_t last = rem ? (mw[full] & (0xFFFFFFFFu >> (8 * (4 - rem)))) : 0u;
+        last |= 0x1Fu << (8 * rem);
+        put(q, last);
+        s[16] ^= half(0x80000000u, 33);
+        keccak_f_il_pair(s);
+#pragma unroll
+        for (int l = 0; l < (OUTB + 7) / 8; l++) {
+            const uint32_t other = pair_swap(s[l]);
+            uint32_t lo, hi;
+            il_from(lane_sel(msk, other, s[l]), lane_sel(msk, s[l], other), lo, hi);
+#pragma unroll
+            for (int o = 0; o < 8; o++)
+                if (8 * l + o < OUTB) out[8 * l + o] = (uint8_t)((o < 4 ? lo : hi) >> (8 * (o & 3)));
+        }
+    }
+
+    // PAIR: run by a lane pair holding identical inputs (stream_msg_pair)
     template <bool PAIR = false>
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
                                    const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
@@ -506,7 +566,8 @@ struct Shake {
             pre[i] = skprf[i];
             pre[NW + i] = opt[i];
         }
-        stream_msg<2 * NW, N>(pre, m, mlen, o);
+        if constexpr (PAIR) stream_msg_pair<2 * NW, N>(pre, m, mlen, o);
+        else stream_msg<2 * NW, N>(pre, m, mlen, o);
 #pragma unroll
         for (int i = 0; i < NW; i++)
             R[i] = (uint32_t)o[4 * i] | (uint32_t)o[4 * i + 1] << 8 |
@@ -521,7 +582,8 @@ struct Shake {
         for (int i = 0; i < NW; i++) pre[i] = R[i];
 #pragma unroll
         for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
-        stream_msg<3 * NW, DG>(pre, m, mlen, dg);
+        if constexpr (PAIR) stream_msg_pair<3 * NW, DG>(pre, m, mlen, dg);
+        else stream_msg<3 * NW, DG>(pre, m, mlen, dg);
     }
 
     template <bool PAIR = false>

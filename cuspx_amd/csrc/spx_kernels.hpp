@@ -100,11 +100,11 @@ __device__ __forceinline__ void plan_from_digest(Plan& pl, const uint8_t* dg) {
     pl.leaf = leaf;
 }
 
-// lanes per request of the prep kernels: SHA-256 requests hash on a lane
-// pair (sha256_compress_dup; both lanes compute everything, the even lane
-// stores), SHAKE256 requests on one lane
+// lanes per request of the prep kernels: every request hashes its message
+// on a lane pair (sha256_compress_dup / Shake::stream_msg_pair; both lanes
+// compute everything, the even lane stores)
 template <class P>
-constexpr uint32_t prep_lanes() { return P::hash == kSha2 ? 2u : 1u; }
+constexpr uint32_t prep_lanes() { return 2u; }
 
 // sign: midstate, R = PRF_msg, digest = H_msg, plan
 template <class P>
@@ -1403,14 +1403,14 @@ __global__ void __launch_bounds__(256) k_vspec(BatchArgs A) {
 
 // Speculative verify, part 2: one wave per request walks the d layers
 // (sign.cu:3077-3113): lanes pick each chain's end for the layer's message
-// from the candidates of k_vspec and load the auth path, then lane 0 runs
-// T_len and compute_root (utils.cu:94-135).  The serial work per layer is
-// T_len + h' nodes instead of the longest chain + T_len + h' nodes.
+// from the candidates of k_vspec and load the auth path, then lanes 0 and 1
+// run T_len and compute_root (utils.cu:94-135) as a lane pair.  The serial
+// work per layer is T_len + h' nodes instead of the longest chain + T_len +
+// h' nodes.
 template <class P>
 __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
-    constexpr bool SHA2 = P::hash == kSha2;
     constexpr int NW = P::nw;
     constexpr int LEN = P::len;
     constexpr int TH = P::th;
