@@ -254,7 +254,7 @@ struct SetImpl {
         if (launch_fused(A, n, S, s)) {
             prof_mark(s, "k_sign_fused");
             if (!A.fors_count) {
-                hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+                hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(2ull * n, 64)), dim3(64), 0, s, A);
                 prof_mark(s, "k_fors_pk");
             }
         } else {
@@ -262,7 +262,7 @@ struct SetImpl {
                                0, s, A);
             prof_mark(s, "k_fors");
             if (!A.fors_count) {
-                hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+                hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(2ull * n, 64)), dim3(64), 0, s, A);
                 prof_mark(s, "k_fors_pk");
             }
             launch_ht<0>(A, n * P::d, S, s);
@@ -297,7 +297,7 @@ struct SetImpl {
         prof_mark(s, "k_vprep");
         hipLaunchKernelGGL((k_vfors<P>), dim3(cdiv((uint64_t)n * P::k, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_vfors");
-        hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(n, 64)), dim3(64), 0, s, A);
+        hipLaunchKernelGGL((k_fors_pk<P>), dim3(cdiv(2ull * n, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_fors_pk");
         if (G == 1) {
             hipLaunchKernelGGL((k_vspec<P>), dim3(cdiv((uint64_t)n * P::d * P::len * 8, 256)),

@@ -5,7 +5,7 @@
 //
 //   sign:   k_prep       lane per request: midstate, PRF_msg, H_msg, plan
 //           k_fors       L lanes per FORS tree (2^J leaves per lane), LDS reduce
-//           k_fors_pk    lane per request: T_k over the k roots (small
+//           k_fors_pk    lane pair per request: T_k over the k roots (small
 //                        batches: run by the lane storing the last root)
 //           k_ht         lane per WOTS leaf of every hypertree layer (all d
 //                        layers at once: every layer's tree/leaf comes from
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(64) k_kprep(BatchArgs A) {
 
 // ------------------------------------------------------------------ FORS
 
-template <class P>
+template <class P, bool PAIR>
 __device__ __forceinline__ void fors_pk_of(const BatchArgs& A, uint32_t i);
 
 // a FORS root of request req is stored.  Small sign batches (fors_count
@@ -260,7 +260,7 @@ __device__ __forceinline__ void fors_root_done(const BatchArgs& A, uint32_t req)
     __threadfence();  // release this root before counting it
     if (atomicAdd(&A.fors_count[req], 1u) == (uint32_t)P::k - 1) {
         __threadfence();  // acquire: every root of the request is visible
-        fors_pk_of<P>(A, req);
+        fors_pk_of<P, false>(A, req);
     }
 }
 
@@ -643,34 +643,54 @@ __device__ __forceinline__ void thash_shake_pair(const typename Shake<P::n>::Key
 }
 
 
-// roots are read past the caches, as another workgroup may have stored them
-template <class P>
+// roots are read past the caches, as another workgroup may have stored them.
+// PAIR: on a lane pair (both lanes active, the even lane stores).
+template <class P, bool PAIR>
 __device__ __forceinline__ void fors_pk_of(const BatchArgs& A, uint32_t i) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     constexpr int NW = P::nw;
+    constexpr int KW = P::k * NW;
     const Plan& pl = A.plan[i];
     Key key;
     Hh::load_key(pl, key);
     Adr adr = adr_make(0, pl.tree, kForsPk);
     adr_keypair<P::th>(adr, pl.leaf);
     const uint32_t* rw = reinterpret_cast<const uint32_t*>(A.fors_roots + (size_t)i * P::k * P::n);
+    auto root = [&](int u) { return __builtin_nontemporal_load(rw + u); };
+    const int h = threadIdx.x & 1;
     uint32_t out[NW];
-    if constexpr (P::hash == kSha2)
-        thash_sha2_rd<NW, P::k * NW>(
-            key.mid, adr, [&](int u) { return bswap32(__builtin_nontemporal_load(rw + u)); }, out);
-    else
-        thash_shake_rd<P, P::k * NW, false>(
-            key, adr, [&](int u) { return __builtin_nontemporal_load(rw + u); }, out);
-    store_val<P::hash, NW>(A.fors_pk + (size_t)i * P::n, out);
+    if constexpr (P::hash == kSha2) {
+        if constexpr (PAIR) {
+            thash_sha2_pair<NW, KW>(key.mid, adr, [&](int j) {
+                const int u = j + h;
+                return bswap32(root(u < 0 ? 0 : u < KW ? u : KW - 1));
+            }, out);
+        } else {
+            thash_sha2_rd<NW, KW>(key.mid, adr, [&](int u) { return bswap32(root(u)); }, out);
+        }
+    } else {
+        if constexpr (PAIR) {
+            thash_shake_pair<P, KW>(key, adr, [&](int j) {  // this lane's half of root lane j / 2
+                int l = j >> 1;
+                l = l < 0 ? 0 : l < KW / 2 ? l : KW / 2 - 1;
+                uint32_t e, o;
+                il_to(root(2 * l), root(2 * l + 1), e, o);
+                return h ? o : e;
+            }, out);
+        } else {
+            thash_shake_rd<P, KW, false>(key, adr, root, out);
+        }
+    }
+    if (!PAIR || h == 0) store_val<P::hash, NW>(A.fors_pk + (size_t)i * P::n, out);
 }
 
-// FORS public keys of a whole batch, lane per request (large sign batches
-// and verify)
+// FORS public keys of a whole batch, a lane pair per request (large sign
+// batches and verify)
 template <class P>
 __global__ void __launch_bounds__(64) k_fors_pk(BatchArgs A) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < A.n_req) fors_pk_of<P>(A, i);
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+    if (i < A.n_req) fors_pk_of<P, true>(A, i);
 }
 
 // verify: one FORS tree per lane: leaf from the revealed sk, climb with the
