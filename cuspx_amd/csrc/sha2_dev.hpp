@@ -305,7 +305,7 @@ struct Sha2 {
     // ------------------------------------------------ per-request hashing
 
     // midstate: compress(IV, pub_seed || 0^(64-n)) (seed_state, sha256.cu:391)
-    // PAIR (here and below): run by a lane pair holding identical inputs
+    // PAIR: run by a lane pair holding identical inputs (the prep kernels)
     template <bool PAIR = false>
     __device__ static __forceinline__ void midstate(const uint32_t pub[NW], uint32_t mid[8]) {
         uint32_t w[16];
@@ -316,12 +316,15 @@ struct Sha2 {
         sha256_compress_by<PAIR>(mid, w);
     }
 
+    // The message hashes run on a lane pair holding identical inputs (the
+    // prep kernels give every request two lanes; sha256_compress_dup).
+    //
     // SHA-256 continuing from st over (pre[0..NPRE) || m[0..mlen)), where
     // `before` bytes were already absorbed.  m is 4-byte aligned; bytes past
     // mlen are never read beyond the word holding the last byte.  Blocks made
     // only of whole message words are loaded directly; the head (prefix) and
     // the tail (last partial word, padding, length) go word by word.
-    template <int NPRE, bool PAIR = false>
+    template <int NPRE>
     __device__ static void stream_msg(uint32_t st[8], const uint32_t pre[NPRE],
                                       const uint8_t* m, uint32_t mlen, uint32_t before) {
         const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
@@ -339,7 +342,7 @@ struct Sha2 {
                 const uint32_t* src = mw + (kk - NPRE);
 #pragma unroll
                 for (int i = 0; i < 16; i++) b[i] = bswap32(src[i]);
-                sha256_compress_by<PAIR>(st, b);
+                sha256_compress_dup(st, b);
                 kk += 16;
                 continue;
             }
@@ -366,13 +369,12 @@ struct Sha2 {
                 x = 0;
             }
             put16(b, (int)(kk & 15), x);
-            if ((kk & 15) == 15) sha256_compress_by<PAIR>(st, b);
+            if ((kk & 15) == 15) sha256_compress_dup(st, b);
             kk++;
         }
     }
 
     // R = HMAC-SHA256(sk_prf, optrand || m)[:n]  (hash_sha256.cu:99-144)
-    template <bool PAIR = false>
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
                                    const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
         uint32_t st[8], w[16], inner[8];
@@ -380,25 +382,25 @@ struct Sha2 {
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
 #pragma unroll
         for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x36363636u;
-        sha256_compress_by<PAIR>(st, w);
-        stream_msg<NW, PAIR>(st, opt, m, mlen, 64);
+        sha256_compress_dup(st, w);
+        stream_msg<NW>(st, opt, m, mlen, 64);
 #pragma unroll
         for (int i = 0; i < 8; i++) inner[i] = st[i];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
 #pragma unroll
         for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x5c5c5c5cu;
-        sha256_compress_by<PAIR>(st, w);
+        sha256_compress_dup(st, w);
 #pragma unroll
         for (int i = 0; i < 16; i++)
             w[i] = i < 8 ? inner[i] : (i == 8 ? 0x80000000u : (i == 15 ? 768u : 0u));
-        sha256_compress_by<PAIR>(st, w);
+        sha256_compress_dup(st, w);
 #pragma unroll
         for (int i = 0; i < NW; i++) R[i] = st[i];
     }
 
     // digest bytes = MGF1(SHA256(R || pk || m), DG)  (hash_sha256.cu:200-247)
-    template <int DG, bool PAIR = false>
+    template <int DG>
     __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
                                  const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
         uint32_t pre[3 * NW], st[8], seed[8], w[16];
@@ -408,7 +410,7 @@ struct Sha2 {
         for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
-        stream_msg<3 * NW, PAIR>(st, pre, m, mlen, 0);
+        stream_msg<3 * NW>(st, pre, m, mlen, 0);
 #pragma unroll
         for (int i = 0; i < 8; i++) seed[i] = st[i];
 #pragma unroll
@@ -420,7 +422,7 @@ struct Sha2 {
                                        : (i == 9 ? 0x80000000u : (i == 15 ? 288u : 0u)));
 #pragma unroll
             for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
-            sha256_compress_by<PAIR>(st, w);
+            sha256_compress_dup(st, w);
 #pragma unroll
             for (int i = 0; i < 32; i++)
                 if (32 * c + i < DG) dg[32 * c + i] = (uint8_t)(st[i >> 2] >> (24 - 8 * (i & 3)));

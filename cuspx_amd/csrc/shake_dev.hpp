@@ -331,12 +331,6 @@ __device__ __forceinline__ void keccak_f_il_pair(uint32_t s[25]) {
         keccak_round_il_pair(s, m, lane_sel(m, kKeccakRcIl.w[2 * r + 1], kKeccakRcIl.w[2 * r]));
 }
 
-// xor x into s[q] for a wave-uniform runtime q < 34 (the rate)
-__device__ __forceinline__ void xor_at(uint32_t s[50], int q, uint32_t x) {
-#pragma unroll
-    for (int i = 0; i < 34; i++) s[i] ^= (q == i) ? x : 0u;
-}
-
 template <int N>
 struct Shake {
     static constexpr int hash = kShake;
@@ -453,51 +447,14 @@ struct Shake {
 
     // ------------------------------------------------ per-request hashing
 
-    // SHAKE256(pre || m) squeezed to OUTB bytes (OUTB <= 136).  m is 4-byte
-    // aligned; the word holding the last message byte is the last one read.
-    // Whole 136-byte blocks of message words are xored in directly; the head
-    // after the prefix and the tail go word by word.
-    template <int NPRE, int OUTB>
-    __device__ static void stream_msg(const uint32_t pre[NPRE], const uint8_t* m, uint32_t mlen,
-                                      uint8_t out[OUTB]) {
-        static_assert(NPRE < 34 && OUTB <= 136, "prefix within one block");
-        const uint32_t* mw = reinterpret_cast<const uint32_t*>(m);
-        uint32_t s[50];
-#pragma unroll
-        for (int i = 0; i < 50; i++) s[i] = i < NPRE ? pre[i] : 0u;
-        int q = NPRE;
-        const uint32_t full = mlen / 4, rem = mlen & 3;
-        uint32_t i = 0;
-#pragma unroll 1
-        while (i < full) {
-            if (q == 0 && i + 34 <= full) {
-#pragma unroll
-                for (int w = 0; w < 34; w++) s[w] ^= mw[i + w];
-                keccak_f(s);
-                i += 34;
-                continue;
-            }
-            xor_at(s, q, mw[i]);
-            q++;
-            i++;
-            if (q == 34) {
-                keccak_f(s);
-                q = 0;
-            }
-        }
-        uint32_t last = rem ? (mw[full] & (0xFFFFFFFFu >> (8 * (4 - rem)))) : 0u;
-        last |= 0x1Fu << (8 * rem);
-        xor_at(s, q, last);
-        s[33] ^= 0x80000000u;
-        keccak_f(s);
-#pragma unroll
-        for (int o = 0; o < OUTB; o++) out[o] = (uint8_t)(s[o >> 2] >> (8 * (o & 3)));
-    }
-
-    // stream_msg on a lane pair (both lanes active with identical inputs;
-    // keccak_f_il_pair): the state is held interleaved, this lane's half of
-    // every Keccak lane, and each absorbed plain word is split into its even
-    // or odd bits on the way in.  The output is in both lanes.
+    // SHAKE256(pre || m) squeezed to OUTB bytes (OUTB <= 136), on a lane pair
+    // (both lanes active with identical inputs; keccak_f_il_pair).  m is
+    // 4-byte aligned; the word holding the last message byte is the last one
+    // read.  The state is held interleaved, this lane's half of every Keccak
+    // lane; each absorbed plain word is split into its even or odd bits on
+    // the way in.  Whole 136-byte blocks of message words are absorbed
+    // directly, the head after the prefix and the tail word by word.  The
+    // output is in both lanes.
     template <int NPRE, int OUTB>
     __device__ static void stream_msg_pair(const uint32_t pre[NPRE], const uint8_t* m,
                                            uint32_t mlen, uint8_t out[OUTB]) {
@@ -555,8 +512,8 @@ struct Shake {
         }
     }
 
-    // PAIR: run by a lane pair holding identical inputs (stream_msg_pair)
-    template <bool PAIR = false>
+    // R = SHAKE256(sk_prf || optrand || m, n) (hash_shake256.cu:64-75), on a
+    // lane pair holding identical inputs
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
                                    const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
         uint32_t pre[2 * NW];
@@ -566,15 +523,16 @@ struct Shake {
             pre[i] = skprf[i];
             pre[NW + i] = opt[i];
         }
-        if constexpr (PAIR) stream_msg_pair<2 * NW, N>(pre, m, mlen, o);
-        else stream_msg<2 * NW, N>(pre, m, mlen, o);
+        stream_msg_pair<2 * NW, N>(pre, m, mlen, o);
 #pragma unroll
         for (int i = 0; i < NW; i++)
             R[i] = (uint32_t)o[4 * i] | (uint32_t)o[4 * i + 1] << 8 |
                    (uint32_t)o[4 * i + 2] << 16 | (uint32_t)o[4 * i + 3] << 24;
     }
 
-    template <int DG, bool PAIR = false>
+    // digest = SHAKE256(R || pk || m, DG) (hash_shake256.cu:96-130), on a
+    // lane pair holding identical inputs
+    template <int DG>
     __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
                                  const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
         uint32_t pre[3 * NW];
@@ -582,10 +540,10 @@ struct Shake {
         for (int i = 0; i < NW; i++) pre[i] = R[i];
 #pragma unroll
         for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
-        if constexpr (PAIR) stream_msg_pair<3 * NW, DG>(pre, m, mlen, dg);
-        else stream_msg<3 * NW, DG>(pre, m, mlen, dg);
+        stream_msg_pair<3 * NW, DG>(pre, m, mlen, dg);
     }
 
+    // no midstate for SHAKE256 (zeros); PAIR as for Sha2::midstate
     template <bool PAIR = false>
     __device__ static __forceinline__ void midstate(const uint32_t*, uint32_t mid[8]) {
 #pragma unroll

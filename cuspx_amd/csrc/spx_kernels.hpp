@@ -103,21 +103,19 @@ __device__ __forceinline__ void plan_from_digest(Plan& pl, const uint8_t* dg) {
 // lanes per request of the prep kernels: every request hashes its message
 // on a lane pair (sha256_compress_dup / Shake::stream_msg_pair; both lanes
 // compute everything, the even lane stores)
-template <class P>
-constexpr uint32_t prep_lanes() { return 2u; }
+constexpr uint32_t kPrepLanes = 2;
 
 // sign: midstate, R = PRF_msg, digest = H_msg, plan
 template <class P>
 __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
     using Hh = HashT<P>;
     constexpr int NW = P::nw;
-    constexpr bool PAIR = prep_lanes<P>() == 2;
     // one serial hash chain per request over the whole message: ask the SIMD
     // arbiter to favour these waves over concurrent bulk hypertree waves
     __builtin_amdgcn_s_setprio(3);
-    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / prep_lanes<P>();
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kPrepLanes;
     if (i >= A.n_req) return;
-    const bool lead = !PAIR || (threadIdx.x & 1u) == 0;
+    const bool lead = (threadIdx.x & 1u) == 0;
     const uint32_t row = A.key_idx ? A.key_idx[i] : i;
     const uint8_t* sk = A.sk + (size_t)row * A.sk_stride;
     uint32_t skseed[NW], skprf[NW], pub[NW], pk[2 * NW], opt[NW], R[NW];
@@ -133,7 +131,7 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
 #pragma unroll
         for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
     } else {
-        Hh::template midstate<PAIR>(pub, pl.mid);
+        Hh::template midstate<true>(pub, pl.mid);
     }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
@@ -141,9 +139,9 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
         pl.pubseed[j] = j < NW ? pub[j] : 0u;
         pl.root[j] = 0;
     }
-    Hh::template prf_msg<PAIR>(skprf, opt, m, ml, R);
+    Hh::prf_msg(skprf, opt, m, ml, R);
     uint8_t dg[P::dgst_bytes];
-    Hh::template h_msg<P::dgst_bytes, PAIR>(R, pk, m, ml, dg);
+    Hh::template h_msg<P::dgst_bytes>(R, pk, m, ml, dg);
     if (!lead) return;
     uint8_t* sig = A.sig + (size_t)i * A.sig_stride;
     store_val<P::hash, NW>(sig, R);
@@ -163,9 +161,8 @@ template <class P>
 __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
     using Hh = HashT<P>;
     constexpr int NW = P::nw;
-    constexpr bool PAIR = prep_lanes<P>() == 2;
     __builtin_amdgcn_s_setprio(3);  // serial message hash per request, as in k_prep
-    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / prep_lanes<P>();
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kPrepLanes;
     if (i >= A.n_req) return;
     const uint32_t row = A.key_idx ? A.key_idx[i] : i;
     const uint8_t* pkp = A.pk + (size_t)row * A.pk_stride;
@@ -179,7 +176,7 @@ __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
 #pragma unroll
         for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
     } else {
-        Hh::template midstate<PAIR>(pub, pl.mid);
+        Hh::template midstate<true>(pub, pl.mid);
     }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
@@ -188,8 +185,8 @@ __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
         pl.root[j] = j < NW ? pk[NW + j] : 0u;
     }
     uint8_t dg[P::dgst_bytes];
-    Hh::template h_msg<P::dgst_bytes, PAIR>(R, pk, A.msg + A.moff[i], A.mlen[i], dg);
-    if (PAIR && (threadIdx.x & 1u)) return;
+    Hh::template h_msg<P::dgst_bytes>(R, pk, A.msg + A.moff[i], A.mlen[i], dg);
+    if (threadIdx.x & 1u) return;  // the even lane of the pair stores
     plan_from_digest<P>(pl, dg);
     pl.status = 0;
     A.plan[i] = pl;
