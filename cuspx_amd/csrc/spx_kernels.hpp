@@ -639,7 +639,7 @@ __device__ __forceinline__ void thash_shake_pair(const typename Shake<P::n>::Key
     Hh::from_il(il, out);
 }
 
-
+// FORS public key of request i: T_k over its k roots (fors.cu:142); the
 // roots are read past the caches, as another workgroup may have stored them.
 // PAIR: on a lane pair (both lanes active, the even lane stores).
 template <class P, bool PAIR>
