@@ -83,6 +83,28 @@ def test_sha2_128f_trace_roots_and_lengths_edge():
     assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * len(lens)
 
 
+@pytest.mark.parametrize("name,lens", [
+    # n = 16: PRF_msg absorbs 32 prefix bytes, H_msg 48, so their first
+    # 136-byte blocks end after 104 / 88 message bytes; whole blocks follow
+    ("shake256-128f", [0, 3, 4, 87, 88, 89, 103, 104, 105, 135, 136, 137, 223, 224, 225,
+                       239, 240, 241, 511, 65536]),
+    # n = 32: prefixes of 64 / 96 bytes, first blocks end after 72 / 40
+    ("shake256-256f", [1, 39, 40, 41, 71, 72, 73, 175, 176, 177, 208, 4097]),
+])
+def test_shake_message_lengths_edge(name, lens):
+    """Message lengths around the SHAKE256 rate-block boundaries of PRF_msg and
+    H_msg (the prep kernels absorb on a lane pair, whole blocks directly and
+    the head and tail word by word): every signature must match the oracle
+    and verify."""
+    pid = C.param_id(name)
+    seeds, msgs, opts = _inputs(pid, len(lens), 78, lens)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    for i in range(len(lens)):
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), lens[i]
+    assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * len(lens)
+
+
 def test_api_surface_matches_oracle_debug_mode():
     # as shipped (DEBUG_MODE): keygen seed 0..3n-1, optrand 0xad (sign.cu:345-349, 1950-1954)
     pid = C.param_id("sha2-128f")
