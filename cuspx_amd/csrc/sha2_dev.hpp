@@ -79,9 +79,10 @@ __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) 
 // stream: Sigma1(e) and Ch(e, f, g) on the even lane, Sigma0(a) and
 // Maj(a, b, c) = Ch(~(a ^ b), b, c) on the odd one (per-lane rotate counts),
 // then a quad_perm DPP swap gives the odd lane T1 and the even lane d.  The
-// schedule makes a word pair per step.  12 instructions per round and 14
+// schedule makes a word pair per step.  11 instructions per round and 14
 // per schedule pair against 14 and 20 on one lane: for serial hashes whose
-// lane would otherwise run alone (T_len and Merkle climbs in verification).
+// lane would otherwise run alone (T_len and Merkle climbs of verification,
+// the parked T_len of signing, the message hashes of the prep kernels).
 
 __device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32_t vin[8]) {
     const bool odd = threadIdx.x & 1u;

@@ -278,9 +278,9 @@ __device__ __forceinline__ void keccak_f_il(uint32_t s[50]) {
 // lane's half of Keccak lane i).  A rotation by 2k rotates both halves by k
 // (no exchange); a rotation by 2k + 1 swaps them (one DPP move) and rotates
 // by k + 1 (even lane) or k (odd lane), a per-lane count.  Theta's column
-// sums, chi and iota stay within a half.  About 108 instructions per round
-// on each lane against 174 for the whole state on one lane: for serial
-// sponges whose lane would otherwise run alone.
+// sums, chi and iota stay within a half.  128 instructions per round on
+// each lane (in the built ISA) against 174 for the whole state on one lane:
+// for serial sponges whose lane would otherwise run alone.
 __device__ __forceinline__ void keccak_round_il_pair(uint32_t s[25], uint32_t m, uint32_t rc) {
     uint32_t C[5];
 #pragma unroll
