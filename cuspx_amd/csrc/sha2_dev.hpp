@@ -129,6 +129,33 @@ __device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32
     s[3] += q3;
 }
 
+// the 64 rounds of sha256_compress_pair with K[t] + W[t] precomputed (kw,
+// read by both lanes; the odd lane masks it out)
+__device__ __forceinline__ void sha256_rounds_pair_kw(uint32_t s[4], const uint32_t* kw) {
+    const bool odd = threadIdx.x & 1u;
+    const uint32_t m = odd ? ~0u : 0u;
+    const uint32_t r1 = odd ? 2u : 6u, r2 = odd ? 13u : 11u, r3 = odd ? 22u : 25u;
+    uint32_t q0 = s[0], q1 = s[1], q2 = s[2], q3 = s[3];
+#pragma unroll
+    for (int t = 0; t < 64; t++) {
+        const uint32_t hk = (q3 + kw[t]) & ~m;
+        const uint32_t sg = xor3(__builtin_amdgcn_alignbit(q0, q0, r1),
+                                 __builtin_amdgcn_alignbit(q0, q0, r2),
+                                 __builtin_amdgcn_alignbit(q0, q0, r3));
+        const uint32_t fn = bit_ch(bit_chi(q0, q1, m), q1, q2);
+        const uint32_t tt = sg + fn + hk;
+        const uint32_t nw = tt + pair_swap(lane_sel(m, q3, tt));
+        q3 = q2;
+        q2 = q1;
+        q1 = q0;
+        q0 = nw;
+    }
+    s[0] += q0;
+    s[1] += q1;
+    s[2] += q2;
+    s[3] += q3;
+}
+
 // sha256_compress on a lane pair whose lanes both hold the whole chaining
 // value st and block w (the message-hash paths): the halves are split for
 // sha256_compress_pair and st is whole again in both lanes afterwards

@@ -439,19 +439,28 @@ __device__ __forceinline__ void thash_sha2_rd(const uint32_t mid[8], const Adr& 
 // thash_sha2_rd on a lane pair (both lanes active with the same mid, a and
 // payload; sha256_compress_pair): rd2(j) returns payload word j + (lane & 1)
 // clamped to 0..M-1, so each lane reads the words of its own half of every
-// block.  The output is in both lanes.
-template <int NW, int M, class Rd2>
-__device__ __forceinline__ void thash_sha2_pair(const uint32_t mid[8], const Adr& a, const Rd2& rd2,
-                                                uint32_t out[NW]) {
-    constexpr int B = (6 + M + 2 + 15) / 16;  // compression blocks
-    constexpr int FT = (M + 5) / 16;          // first block that is not payload only
-    constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
-    const bool odd = threadIdx.x & 1u;
-    const uint32_t m = odd ? ~0u : 0u;
+// block.  The steps are separate so that a caller can feed the payload-only
+// blocks from precomputed K + W (sha256_rounds_pair_kw).
+template <int NW, int M>
+struct Sha2Pair {
+    static constexpr int B = (6 + M + 2 + 15) / 16;  // compression blocks
+    static constexpr int FT = (M + 5) / 16;          // first block that is not payload only
+    static constexpr uint32_t bits = (64u + 22u + 4u * M) * 8u;
+
+    __device__ static __forceinline__ uint32_t mask() { return (threadIdx.x & 1u) ? ~0u : 0u; }
+
+    // chaining value halves: e..h (even lane), a..d (odd lane)
+    __device__ static __forceinline__ void init(const uint32_t mid[8], uint32_t st[4]) {
+        const uint32_t m = mask();
+#pragma unroll
+        for (int i = 0; i < 4; i++) st[i] = lane_sel(m, mid[i], mid[4 + i]);
+    }
     // word pairs of block tb (a constant once unrolled): word g = 16 tb + i is
     // address, payload re-aligned by 2 bytes (lo / hi = this lane's payload
     // words g - 6 and g - 5), the 0x80 pad, zero or the bit length
-    auto block = [&](int tb, uint32_t v[8]) {
+    template <class Rd2>
+    __device__ static __forceinline__ void block(int tb, const Adr& a, const Rd2& rd2, uint32_t v[8]) {
+        const uint32_t m = mask();
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t lo = rd2(16 * tb - 6 + 2 * k), hi = rd2(16 * tb - 5 + 2 * k);
@@ -467,32 +476,73 @@ __device__ __forceinline__ void thash_sha2_pair(const uint32_t mid[8], const Adr
             }
             v[k] = lane_sel(m, y[1], y[0]);
         }
-    };
-    uint32_t st[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) st[i] = lane_sel(m, mid[i], mid[4 + i]);
-#pragma unroll 1
-    for (int blk = 0; blk < B; blk++) {
-        uint32_t v[8];
-        if (blk == 0) {
-            block(0, v);
-        } else if (blk < FT) {  // payload only
-            const int base = 16 * blk - 6;
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                v[k] = __builtin_amdgcn_alignbit(rd2(base + 2 * k), rd2(base + 2 * k + 1), 16);
-        } else {
-#pragma unroll
-            for (int tb = 1; tb < B; tb++)
-                if (tb >= FT && blk == tb) block(tb, v);
-        }
-        sha256_compress_pair(st, v);
     }
-    // a..d are on the odd lane, e..h on the even one
+    // payload-only block blk (runtime): this lane's word pairs
+    template <class Rd2>
+    __device__ static __forceinline__ void payload(int blk, const Rd2& rd2, uint32_t v[8]) {
+        const int base = 16 * blk - 6;
 #pragma unroll
-    for (int w = 0; w < NW; w++) {
-        const uint32_t own = st[w & 3], other = pair_swap(st[w & 3]);
-        out[w] = w < 4 ? lane_sel(m, own, other) : lane_sel(m, other, own);
+        for (int k = 0; k < 8; k++)
+            v[k] = __builtin_amdgcn_alignbit(rd2(base + 2 * k), rd2(base + 2 * k + 1), 16);
+    }
+    // blocks from..to-1
+    template <class Rd2>
+    __device__ static __forceinline__ void absorb(uint32_t st[4], int from, int to, const Adr& a,
+                                                  const Rd2& rd2) {
+#pragma unroll 1
+        for (int blk = from; blk < to; blk++) {
+            uint32_t v[8];
+            if (blk == 0) {
+                block(0, a, rd2, v);
+            } else if (blk < FT) {
+                payload(blk, rd2, v);
+            } else {
+#pragma unroll
+                for (int tb = 1; tb < B; tb++)
+                    if (tb >= FT && blk == tb) block(tb, a, rd2, v);
+            }
+            sha256_compress_pair(st, v);
+        }
+    }
+    // a..d are on the odd lane, e..h on the even one; the output in both
+    __device__ static __forceinline__ void output(const uint32_t st[4], uint32_t out[NW]) {
+        const uint32_t m = mask();
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const uint32_t own = st[w & 3], other = pair_swap(st[w & 3]);
+            out[w] = w < 4 ? lane_sel(m, own, other) : lane_sel(m, other, own);
+        }
+    }
+};
+
+// the whole thash on a lane pair; the output is in both lanes
+template <int NW, int M, class Rd2>
+__device__ __forceinline__ void thash_sha2_pair(const uint32_t mid[8], const Adr& a, const Rd2& rd2,
+                                                uint32_t out[NW]) {
+    using SP = Sha2Pair<NW, M>;
+    uint32_t st[4];
+    SP::init(mid, st);
+    SP::absorb(st, 0, SP::B, a, rd2);
+    SP::output(st, out);
+}
+
+// K[t] + W[t] (t = 0..63) of payload-only block b of a thash_sha2_pair
+// payload held in LDS as native words (pk), on one lane: the message
+// schedule of a block does not depend on the chaining value, so idle lanes
+// can run it ahead of the pair that compresses (sha256_rounds_pair_kw)
+__device__ __forceinline__ void sha2_kw_block(const uint32_t* pk, int b, uint32_t kw[64]) {
+    uint32_t w[16];
+    const int base = 16 * b - 6;
+#pragma unroll
+    for (int i = 0; i < 16; i++) w[i] = __builtin_amdgcn_alignbit(pk[base + i], pk[base + i + 1], 16);
+#pragma unroll
+    for (int t = 0; t < 64; t++) {
+        if (t >= 16) {
+            const uint32_t x = w[(t - 15) & 15], y = w[(t - 2) & 15];
+            w[t & 15] = xor3(rotr32(y, 17), rotr32(y, 19), y >> 10) + w[(t - 7) & 15] +
+                        xor3(rotr32(x, 7), rotr32(x, 18), x >> 3) + w[t & 15];
+        }
+        kw[t] = kSha256K[t] + w[t & 15];
     }
 }
 
@@ -1064,28 +1114,17 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     }
 }
 
-// Verification of one layer's subtree on a lane pair (both lanes active):
-// T_len over the WOTS public key in LDS (pk: native words for SHA-256,
-// interleaved for SHAKE256) and the climb (compute_root, utils.cu:94-135)
-// with the auth path in LDS (raw words); nd is NW words of LDS for the node
-// of the climb.  node is the layer root, in both lanes.
+// The climb of one layer's subtree on a lane pair (compute_root,
+// utils.cu:94-135): node (in both lanes) is the WOTS public key's T_len on
+// entry and the layer root on exit; the auth path is in LDS (raw words) and
+// nd is NW words of LDS for the node.
 template <class P>
-__device__ __forceinline__ void tlen_climb_pair(const typename HashT<P>::Key& key,
-                                                uint32_t layer, uint64_t tree, uint32_t leaf,
-                                                const uint32_t* pk, const uint32_t* auth,
-                                                uint32_t* nd, uint32_t node[P::nw]) {
+__device__ __forceinline__ void climb_pair(const typename HashT<P>::Key& key, uint32_t layer,
+                                           uint64_t tree, uint32_t leaf, const uint32_t* auth,
+                                           uint32_t* nd, uint32_t node[P::nw]) {
     constexpr int NW = P::nw;
-    constexpr int TLW = P::len * NW;
     constexpr bool SHA2 = P::hash == kSha2;
     const int half = threadIdx.x & 1;
-    Adr pa = adr_make(layer, tree, kWotsPk);
-    adr_keypair<P::th>(pa, leaf);
-    auto rpk = [&](int j) {
-        const int i = j + half;
-        return pk[i < 0 ? 0 : i < TLW ? i : TLW - 1];
-    };
-    if constexpr (SHA2) thash_sha2_pair<NW, TLW>(key.mid, pa, rpk, node);
-    else thash_shake_pair<P, TLW>(key, pa, rpk, node);
     uint32_t li = leaf;
     Adr ha = adr_make(layer, tree, kHashTree);
 #pragma unroll 1
@@ -1121,6 +1160,37 @@ __device__ __forceinline__ void tlen_climb_pair(const typename HashT<P>::Key& ke
     }
 }
 
+// reader of a WOTS public key held in LDS for the lane-pair thashes
+template <class P>
+struct PkPair {
+    static constexpr int TLW = P::len * P::nw;
+    const uint32_t* pk;
+    int half;
+    __device__ uint32_t operator()(int j) const {
+        const int i = j + half;
+        return pk[i < 0 ? 0 : i < TLW ? i : TLW - 1];
+    }
+};
+
+// Verification of one layer's subtree on a lane pair (both lanes active):
+// T_len over the WOTS public key in LDS (pk: native words for SHA-256,
+// interleaved for SHAKE256), then climb_pair.  node is the layer root, in
+// both lanes.
+template <class P>
+__device__ __forceinline__ void tlen_climb_pair(const typename HashT<P>::Key& key,
+                                                uint32_t layer, uint64_t tree, uint32_t leaf,
+                                                const uint32_t* pk, const uint32_t* auth,
+                                                uint32_t* nd, uint32_t node[P::nw]) {
+    constexpr int NW = P::nw;
+    constexpr int TLW = P::len * NW;
+    Adr pa = adr_make(layer, tree, kWotsPk);
+    adr_keypair<P::th>(pa, leaf);
+    const PkPair<P> rpk{pk, (int)(threadIdx.x & 1)};
+    if constexpr (P::hash == kSha2) thash_sha2_pair<NW, TLW>(key.mid, pa, rpk, node);
+    else thash_shake_pair<P, TLW>(key, pa, rpk, node);
+    climb_pair<P>(key, layer, tree, leaf, auth, nd, node);
+}
+
 // verify: the d layers of R = 256 / G requests per workgroup in order
 // (sign.cu:3077-3113), so G is the mean number of lanes per request.  Per
 // layer the R * len WOTS chains of the workgroup (15 - len_j steps each,
@@ -1149,6 +1219,10 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     constexpr int AW = TH * NW;                  // auth path words per request
     constexpr int MAXA = (R * AW + 255) / 256;   // prefetched auth words per lane
     static_assert(NCH < 4096, "chain ids take 12 bits of an order entry");
+    // SHA-256 with n <= 24: T_len's payload-only blocks take their message
+    // schedules from waves 1-3 (the LDS for n = 32 would not fit)
+    constexpr int MB = (TLW + 5) / 16 - 1;  // payload-only blocks of T_len
+    constexpr bool PRE = SHA2 && NW <= 6;
     __builtin_amdgcn_s_setprio(2);  // d dependent layers per request: latency-bound
     __shared__ uint32_t key_s[R][8];    // SHA-256 midstate / SHAKE256 pub_seed
     __shared__ uint32_t root_s[R][NW];  // root of the last finished layer
@@ -1158,6 +1232,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     __shared__ uint32_t pk_s[R][TLW];   // chain ends (SHAKE256: interleaved)
     __shared__ uint32_t auth_s[R][AW];  // auth path of the layer, raw words
     __shared__ uint32_t node_s[R][NW];  // climb: the node, raw words
+    __shared__ uint32_t kw_s[PRE ? R * MB : 1][64];  // K + W of T_len's payload-only blocks
     __shared__ uint16_t order_s[NCH];   // rank -> chain id | remaining steps << 12
     __shared__ uint32_t hist_s[16];     // chains per remaining-step count
 
@@ -1311,7 +1386,50 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
         // T_len and the subtree climb of request rq on the lane pair 2 rq,
         // 2 rq + 1 (sha256_compress_pair / keccak_f_il_pair)
         const int rq = tid >> 1;
-        if (tid < 2 * nact) {
+        if constexpr (PRE) {
+            // SHA-256: waves 1-3 run the message schedules of T_len's
+            // payload-only blocks (they depend on the chain ends only) while
+            // the pairs of wave 0 compress block 0; the pairs then take
+            // K + W of those blocks from LDS
+            using SP = Sha2Pair<NW, TLW>;
+            uint32_t st[4];
+            Key k2;
+            uint64_t tree = 0;
+            uint32_t leaf = 0;
+            Adr pa{};
+            const PkPair<P> rpk{&pk_s[rq < R ? rq : 0][0], tid & 1};
+            if (tid < 2 * nact) {
+                tree = tree_s[rq];
+                leaf = leaf_s[rq];
+#pragma unroll
+                for (int i = 0; i < NW; i++) k2.sk[i] = 0;
+#pragma unroll
+                for (int i = 0; i < 8; i++) k2.mid[i] = key_s[rq][i];
+                pa = adr_make(layer, tree, kWotsPk);
+                adr_keypair<TH>(pa, leaf);
+                SP::init(k2.mid, st);
+                SP::absorb(st, 0, 1, pa, rpk);
+            }
+            if (tid >= 64) {
+                for (int it = tid - 64; it < nact * MB; it += 192)
+                    sha2_kw_block(&pk_s[it / MB][0], 1 + it % MB, &kw_s[it][0]);
+            }
+            __syncthreads();  // K + W of every payload-only block in LDS
+            if (tid < 2 * nact) {
+#pragma unroll 1
+                for (int b = 0; b < MB; b++) sha256_rounds_pair_kw(st, &kw_s[rq * MB + b][0]);
+                SP::absorb(st, SP::FT, SP::B, pa, rpk);
+                uint32_t node[NW];
+                SP::output(st, node);
+                climb_pair<P>(k2, layer, tree, leaf, &auth_s[rq][0], &node_s[rq][0], node);
+                if ((tid & 1) == 0) {
+#pragma unroll
+                    for (int w = 0; w < NW; w++) root_s[rq][w] = node[w];
+                    leaf_s[rq] = (uint32_t)(tree & ((1u << TH) - 1));
+                    tree_s[rq] = tree >> TH;
+                }
+            }
+        } else if (tid < 2 * nact) {
             const uint64_t tree = tree_s[rq];
             const uint32_t leaf = leaf_s[rq];
             Key k2;
