@@ -1539,7 +1539,8 @@ __global__ void __launch_bounds__(256) k_vspec(BatchArgs A) {
 // Speculative verify, part 2: one wave per request walks the d layers
 // (sign.cu:3077-3113): lanes pick each chain's end for the layer's message
 // from the candidates of k_vspec and load the auth path, then lanes 0 and 1
-// run T_len and compute_root (utils.cu:94-135) as a lane pair.  The serial
+// run T_len and compute_root (utils.cu:94-135) as a lane pair (SHA-256: with
+// the message schedules of T_len's payload-only blocks from lanes 2..).  The serial
 // work per layer is T_len + h' nodes instead of the longest chain + T_len +
 // h' nodes.
 template <class P>
@@ -1555,6 +1556,10 @@ __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
     __shared__ uint32_t auth_s[AW];   // auth path, raw words
     __shared__ uint32_t root_s[NW];
     __shared__ uint32_t node_s[NW];   // climb: the node, raw words
+    // SHA-256: K + W of T_len's payload-only blocks
+    constexpr int MB = P::hash == kSha2 ? (TLW + 5) / 16 - 1 : 1;
+    static_assert(MB <= 62, "one lane per block");
+    __shared__ uint32_t kw_s[MB][64];
     const int tid = threadIdx.x;
     const uint32_t req = blockIdx.x;  // grid = n_req
     const Plan& pl = A.plan[req];
@@ -1585,15 +1590,36 @@ __global__ void __launch_bounds__(64) k_vht_spec(BatchArgs A) {
             sig + P::n + P::fors_bytes + (size_t)layer * P::layer_bytes + P::wots_bytes);
         for (int u = tid; u < AW; u += 64) auth_s[u] = auth[u];
         __syncthreads();
-        if (tid < 2) {  // T_len and the climb on the lane pair 0, 1
+        uint32_t node[NW];
+        if constexpr (P::hash == kSha2) {
+            // lanes 2.. compute K + W of T_len's payload-only blocks (one
+            // pass for all of them), then the pair runs only their rounds
+            if (tid >= 2 && tid < 2 + MB) sha2_kw_block(pk_s, tid - 1, kw_s[tid - 2]);
+            __syncthreads();
+            if (tid < 2) {  // T_len and the climb on the lane pair 0, 1
+                using SP = Sha2Pair<NW, TLW>;
+                Key key;
+                Hh::load_key(pl, key);
+                Adr pa = adr_make(layer, tree, kWotsPk);
+                adr_keypair<TH>(pa, leaf);
+                const PkPair<P> rpk{pk_s, tid};
+                uint32_t st[4];
+                SP::init(key.mid, st);
+                SP::absorb(st, 0, 1, pa, rpk);
+#pragma unroll 1
+                for (int b = 0; b < MB; b++) sha256_rounds_pair_kw(st, kw_s[b]);
+                SP::absorb(st, SP::FT, SP::B, pa, rpk);
+                SP::output(st, node);
+                climb_pair<P>(key, layer, tree, leaf, auth_s, node_s, node);
+            }
+        } else if (tid < 2) {  // T_len and the climb on the lane pair 0, 1
             Key key;
             Hh::load_key(pl, key);
-            uint32_t node[NW];
             tlen_climb_pair<P>(key, layer, tree, leaf, pk_s, auth_s, node_s, node);
-            if (tid == 0) {
+        }
+        if (tid == 0) {
 #pragma unroll
-                for (int w = 0; w < NW; w++) root_s[w] = node[w];
-            }
+            for (int w = 0; w < NW; w++) root_s[w] = node[w];
         }
         leaf = (uint32_t)(tree & ((1u << TH) - 1));
         tree >>= TH;
