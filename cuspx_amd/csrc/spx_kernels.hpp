@@ -1198,11 +1198,13 @@ __device__ __forceinline__ void tlen_climb_pair(const typename HashT<P>::Key& ke
 // and dealt to the 256 lanes in snake order: a longest-first schedule that
 // evens the per-lane F counts across all requests of the workgroup, not per
 // request.  Each lane runs its chains in one flattened loop (one F site;
-// SHAKE256 chains run bit-interleaved).  Lane r < R then computes T_len of
-// request r block by block from LDS and climbs its subtree (compute_root,
-// utils.cu:94-135).  The next layer's WOTS signature words and this layer's
-// auth path are fetched into registers while the chains run, so the layer
-// loop waits on no global load.
+// SHAKE256 chains run bit-interleaved).  The lane pair 2r, 2r + 1 then
+// computes T_len of request r block by block from LDS and climbs its subtree
+// (compute_root, utils.cu:94-135; tlen_climb_pair, or for SHA-256 with n <=
+// 24 the rounds of T_len's payload-only blocks on K + W that waves 1-3
+// computed meanwhile).  The next layer's WOTS signature words and this
+// layer's auth path are fetched into registers while the chains run, so the
+// layer loop waits on no global load.
 template <class P, int G>
 __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     using Hh = HashT<P>;
