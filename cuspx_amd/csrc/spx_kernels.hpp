@@ -3,7 +3,7 @@
 // One signature is split across many lanes (MI355X needs >> 256 workgroups
 // and prefers uniform control flow inside a wave):
 //
-//   sign:   k_prep       lane per request: midstate, PRF_msg, H_msg, plan
+//   sign:   k_prep       lane pair per request: midstate, PRF_msg, H_msg, plan
 //           k_fors       L lanes per FORS tree (2^J leaves per lane), LDS reduce
 //           k_fors_pk    lane pair per request: T_k over the k roots (small
 //                        batches: run by the lane storing the last root)
@@ -923,7 +923,8 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     Adr pa = adr_make(layer, tree, kWotsPk);
     adr_keypair<P::th>(pa, u);
     // SHA-256 at S = 1 streams T_len in registers; otherwise the chain ends
-    // are parked in tl_buf and one lane per leaf absorbs them afterwards
+    // are parked in tl_buf and absorbed afterwards (by the leaf's lane pair
+    // when S > 1)
     typename Sha2<P::n>::Stream ts;
     if constexpr (P::hash == kSha2 && !Sh::park) ts.begin(key, pa);
     auto parked = [&](int w) { return A.tl_buf[(size_t)w * A.tl_lanes + leaf_g]; };
