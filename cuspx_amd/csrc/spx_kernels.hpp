@@ -1097,22 +1097,39 @@ __global__ void __launch_bounds__(256) k_wots_gather(BatchArgs A) {
     uint32_t msg[NW];
     load_val<P::hash, NW>(mp, msg);
     const uint32_t len = chain_len<P>(msg, j);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(
-        A.cap + ((((size_t)req * P::d + layer) * P::len + j) * 16 + len) * P::n);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(A.sig + (size_t)req * A.sig_stride + P::n +
-                                                P::fors_bytes + (size_t)layer * P::layer_bytes +
-                                                (size_t)j * P::n);
-    if constexpr (P::hash == kShake) {  // SHAKE chains are captured bit-interleaved
-        uint32_t vi[NW], v[NW];
+    // the capture holds n-byte values at multiples of n from a 256-byte
+    // aligned base: 16-byte loads (8-byte for n = 24), all before the stores
+    // (word-by-word load / store pairs could alias and stayed 4-byte
+    // accesses)
+    const uint8_t* src = A.cap + ((((size_t)req * P::d + layer) * P::len + j) * 16 + len) * P::n;
+    uint8_t* dst = A.sig + (size_t)req * A.sig_stride + P::n + P::fors_bytes +
+                   (size_t)layer * P::layer_bytes + (size_t)j * P::n;
+    uint32_t v[NW];
+    if constexpr (NW % 4 == 0) {
 #pragma unroll
-        for (int w = 0; w < NW; w++) vi[w] = src[w];
-        Shake<P::n>::from_il(vi, v);
-#pragma unroll
-        for (int w = 0; w < NW; w++) dst[w] = v[w];
+        for (int k = 0; k < NW / 4; k++) {
+            const uint4 x = reinterpret_cast<const uint4*>(src)[k];
+            v[4 * k] = x.x;
+            v[4 * k + 1] = x.y;
+            v[4 * k + 2] = x.z;
+            v[4 * k + 3] = x.w;
+        }
     } else {
 #pragma unroll
-        for (int w = 0; w < NW; w++) dst[w] = src[w];
+        for (int k = 0; k < NW / 2; k++) {
+            const uint2 x = reinterpret_cast<const uint2*>(src)[k];
+            v[2 * k] = x.x;
+            v[2 * k + 1] = x.y;
+        }
     }
+    if constexpr (P::hash == kShake) {  // SHAKE chains are captured bit-interleaved
+        uint32_t vi[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) vi[w] = v[w];
+        Shake<P::n>::from_il(vi, v);
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) reinterpret_cast<uint32_t*>(dst)[w] = v[w];
 }
 
 // The climb of one layer's subtree on a lane pair (compute_root,
