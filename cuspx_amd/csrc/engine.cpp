@@ -129,6 +129,8 @@ static int fail(int code, const char* fmt, const char* what = "") {
     return code;
 }
 
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
 #define HIP_OK(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
@@ -137,29 +139,19 @@ static int fail(int code, const char* fmt, const char* what = "") {
 
 // ------------------------------------------------------------ the device
 
-struct Device {
-    std::mutex mu;
-    bool ready = false;
-    int dev = 0;
+// One stream of the host-buffer calls with its own device arena and pinned
+// staging (both grow-only) and an event recorded after its kernels.
+struct Lane {
     hipStream_t stream = nullptr;
-    void* dbuf = nullptr;   // device arena for host-buffer calls (grow-only)
+    hipEvent_t kernels_done = nullptr;
+    void* dbuf = nullptr;
     size_t dcap = 0;
-    uint8_t* hbuf = nullptr;  // pinned staging (grow-only)
+    uint8_t* hbuf = nullptr;
     size_t hcap = 0;
-    size_t budget = 0;      // bytes one host-buffer call may use on the device
 
-    int init() {
-        if (ready) return kOk;
-        HIP_OK(hipGetDevice(&dev));
-        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        size_t fr = 0, tot = 0;
-        HIP_OK(hipMemGetInfo(&fr, &tot));
-        const char* env = getenv("CUSPX_HBM_BUDGET_MB");
-        budget = env ? (size_t)atoll(env) << 20 : fr / 2;
-        ready = true;
-        return kOk;
-    }
     int ensure(size_t dbytes, size_t hbytes) {
+        if (!stream) HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (!kernels_done) HIP_OK(hipEventCreateWithFlags(&kernels_done, hipEventDisableTiming));
         if (dbytes > dcap) {
             if (dbuf) HIP_OK(hipFree(dbuf));
             dbuf = nullptr;
@@ -174,6 +166,31 @@ struct Device {
             HIP_OK(hipHostMalloc((void**)&hbuf, hbytes, hipHostMallocDefault));
             hcap = hbytes;
         }
+        return kOk;
+    }
+};
+
+// Lane 0 serves the single-stream host-buffer calls; the multi-stream ones
+// (face_ms_mdp_*) cut a batch into up to kLanes slices on lanes 0.., so one
+// slice's PCIe copies and host staging overlap another slice's kernels.
+constexpr int kLanes = 4;
+
+struct Device {
+    std::mutex mu;
+    bool ready = false;
+    int dev = 0;
+    Lane lane[kLanes];
+    size_t budget = 0;  // bytes one host-buffer call may use on the device
+
+    int init() {
+        if (ready) return kOk;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipStreamCreateWithFlags(&lane[0].stream, hipStreamNonBlocking));
+        size_t fr = 0, tot = 0;
+        HIP_OK(hipMemGetInfo(&fr, &tot));
+        const char* env = getenv("CUSPX_HBM_BUDGET_MB");
+        budget = env ? (size_t)atoll(env) << 20 : fr / 2;
+        ready = true;
         return kOk;
     }
 };
@@ -297,27 +314,97 @@ static MsgPack pack_msgs(uint32_t num, const uint32_t* mlen) {
     return mp;
 }
 
-// sign `num` items: sk_i = sk + i*sk_stride, message i at msg + moff[i]
-static int sign_host(const SetOps* o, uint32_t num, const uint8_t* sk, uint64_t sk_stride,
-                     const uint8_t* msg, const uint64_t* moff, const uint32_t* mlen,
-                     const uint8_t* optrand, uint8_t* sig, uint64_t sig_stride) {
+// One host-buffer slice in flight on a lane: its inputs are staged and
+// copied, its kernels and the output copy enqueued.  finish_job waits for the
+// lane and moves the outputs into the caller's buffers.
+struct Job {
+    Lane* lane = nullptr;
+    const SetOps* o = nullptr;
+    int op = -1;
+    uint32_t num = 0;
+    uint8_t* out = nullptr;  // sign: signatures; verify: statuses; keygen: pk
+    uint64_t out_stride = 0;
+    uint8_t* out2 = nullptr;  // keygen: sk
+};
+
+static int finish_job(Job& j) {
+    if (!j.lane) return kOk;
+    Lane& L = *j.lane;
+    j.lane = nullptr;
+    HIP_OK(hipStreamSynchronize(L.stream));
+    const SetOps* o = j.o;
+    if (j.op == kOpSign) {
+        copy_items(j.out, j.out_stride, L.hbuf, o->sig_bytes, o->sig_bytes, j.num);
+    } else if (j.op == kOpVerify) {
+        memcpy(j.out, L.hbuf, 4ull * j.num);
+    } else {
+        memcpy(j.out, L.hbuf, (uint64_t)j.num * o->pk_bytes);
+        memcpy(j.out2, L.hbuf + al16((uint64_t)j.num * o->pk_bytes),
+               (uint64_t)j.num * o->sk_bytes);
+    }
+    return kOk;
+}
+
+// Between the input copy and the kernels of a slice: wait for the previous
+// slice's kernels (multi-stream calls run the slices' kernels in order, so
+// each slice finishes early enough for its output copy to overlap the next
+// slice's kernels).  After the kernels: mark them done.
+static int wait_prev(Lane& L, const Lane* prev) {
+    if (prev) HIP_OK(hipStreamWaitEvent(L.stream, prev->kernels_done, 0));
+    return kOk;
+}
+static int mark_done(Lane& L) {
+    HIP_OK(hipEventRecord(L.kernels_done, L.stream));
+    return kOk;
+}
+
+// Runs a host-buffer batch as `slices` consecutive slices on lanes 0..slices-1
+// (one slice: the whole batch on lane 0).  Slice c is staged and enqueued
+// while the slices before it run; every lane is finished at the end, in
+// order, so a slice's outputs are copied out while later slices compute.
+// enq(lane, previous lane or null, first item, items, device bytes allowed,
+// job) enqueues one slice.
+template <class Enq>
+static int run_slices(uint32_t num, int slices, Enq enq) {
     if (num == 0) return kOk;
-    if (!sk || !sig || !mlen || !moff || !optrand) return fail(kErrArg, "null argument%s");
     Device& D = device();
     std::lock_guard<std::mutex> lk(D.mu);
     int rc = D.init();
     if (rc) return rc;
+    slices = slices < 1 ? 1 : slices > kLanes ? kLanes : slices;
+    const uint32_t per = (num + slices - 1) / slices;
+    Job jobs[kLanes];
+    int err = kOk;
+    const Lane* prev = nullptr;
+    for (int c = 0; c < slices && err == kOk; c++) {
+        const uint32_t a = c * per, b = a + per < num ? a + per : num;
+        if (a >= b) break;
+        err = enq(D.lane[c], slices > 1 ? prev : nullptr, a, b - a, D.budget / slices, jobs[c]);
+        prev = &D.lane[c];
+    }
+    for (int c = 0; c < slices; c++) {
+        rc = finish_job(jobs[c]);
+        if (rc && err == kOk) err = rc;
+    }
+    return err;
+}
+
+// sign `num` items on lane L: sk_i = sk + i*sk_stride, message i at msg + moff[i]
+static int sign_enqueue(const SetOps* o, Lane& L, const Lane* prev, size_t budget, uint32_t num,
+                        const uint8_t* sk, uint64_t sk_stride, const uint8_t* msg,
+                        const uint64_t* moff, const uint32_t* mlen, const uint8_t* optrand,
+                        uint8_t* sig, uint64_t sig_stride, Job& job) {
     MsgPack mp = pack_msgs(num, mlen);
     const uint64_t n = o->n;
     const uint64_t in_bytes = al16((uint64_t)num * o->sk_bytes) + mp.bytes + al16(8ull * num) +
                               al16(4ull * num) + al16((uint64_t)num * n);
     const uint64_t out_bytes = al16((uint64_t)num * o->sig_bytes);
     const uint64_t ws = o->workspace(num, kOpSign);
-    if (in_bytes + out_bytes + ws + 512 > D.budget)
+    if (in_bytes + out_bytes + ws + 512 > budget)
         return fail(kErrBudget, "sign batch exceeds the device-memory budget%s");
-    rc = D.ensure(in_bytes + out_bytes + ws + 512, in_bytes > out_bytes ? in_bytes : out_bytes);
+    int rc = L.ensure(in_bytes + out_bytes + ws + 512, in_bytes > out_bytes ? in_bytes : out_bytes);
     if (rc) return rc;
-    uint8_t* h = D.hbuf;
+    uint8_t* h = L.hbuf;
     uint8_t* p = h;
     copy_items(p, o->sk_bytes, sk, sk_stride, o->sk_bytes, num);
     p += al16((uint64_t)num * o->sk_bytes);
@@ -330,8 +417,10 @@ static int sign_host(const SetOps* o, uint32_t num, const uint8_t* sk, uint64_t 
     memcpy(p, mlen, 4ull * num);
     p += al16(4ull * num);
     memcpy(p, optrand, (uint64_t)num * n);
-    uint8_t* d = (uint8_t*)D.dbuf;
-    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, D.stream));
+    uint8_t* d = (uint8_t*)L.dbuf;
+    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, L.stream));
+    rc = wait_prev(L, prev);
+    if (rc) return rc;
     DevBatch b{};
     b.n = num;
     b.sk = d;
@@ -348,34 +437,48 @@ static int sign_host(const SetOps* o, uint32_t num, const uint8_t* sk, uint64_t 
     b.sig = dsig;
     b.sig_stride = o->sig_bytes;
     b.ws = dsig + out_bytes;
-    rc = o->sign(b, D.stream);
+    rc = o->sign(b, L.stream);
     if (rc) return fail(rc, "kernel launch failed%s");
-    HIP_OK(hipMemcpyAsync(h, dsig, (uint64_t)num * o->sig_bytes, hipMemcpyDeviceToHost, D.stream));
-    HIP_OK(hipStreamSynchronize(D.stream));
-    copy_items(sig, sig_stride, h, o->sig_bytes, o->sig_bytes, num);
+    rc = mark_done(L);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(h, dsig, (uint64_t)num * o->sig_bytes, hipMemcpyDeviceToHost, L.stream));
+    job.lane = &L;
+    job.o = o;
+    job.op = kOpSign;
+    job.num = num;
+    job.out = sig;
+    job.out_stride = sig_stride;
     return kOk;
 }
 
-static int verify_host(const SetOps* o, uint32_t num, const uint8_t* pk, uint64_t pk_stride,
-                       const uint8_t* sig, uint64_t sig_stride, const uint8_t* msg,
-                       const uint64_t* moff, const uint32_t* mlen, int32_t* status) {
+static int sign_host(const SetOps* o, uint32_t num, const uint8_t* sk, uint64_t sk_stride,
+                     const uint8_t* msg, const uint64_t* moff, const uint32_t* mlen,
+                     const uint8_t* optrand, uint8_t* sig, uint64_t sig_stride, int slices = 1) {
     if (num == 0) return kOk;
-    if (!pk || !sig || !mlen || !moff || !status) return fail(kErrArg, "null argument%s");
-    Device& D = device();
-    std::lock_guard<std::mutex> lk(D.mu);
-    int rc = D.init();
-    if (rc) return rc;
+    if (!sk || !sig || !mlen || !moff || !optrand) return fail(kErrArg, "null argument%s");
+    return run_slices(num, slices, [&](Lane& L, const Lane* prev, uint32_t a, uint32_t cnt,
+                                       size_t budget, Job& job) {
+        return sign_enqueue(o, L, prev, budget, cnt, sk + a * sk_stride, sk_stride, msg, moff + a,
+                            mlen + a, optrand + (uint64_t)a * o->n, sig + a * sig_stride,
+                            sig_stride, job);
+    });
+}
+
+static int verify_enqueue(const SetOps* o, Lane& L, const Lane* prev, size_t budget,
+                          uint32_t num, const uint8_t* pk, uint64_t pk_stride, const uint8_t* sig,
+                          uint64_t sig_stride, const uint8_t* msg, const uint64_t* moff,
+                          const uint32_t* mlen, int32_t* status, Job& job) {
     MsgPack mp = pack_msgs(num, mlen);
     const uint64_t in_bytes = al16((uint64_t)num * o->pk_bytes) +
                               al16((uint64_t)num * o->sig_bytes) + mp.bytes + al16(8ull * num) +
                               al16(4ull * num);
     const uint64_t out_bytes = al16(4ull * num);
     const uint64_t ws = o->workspace(num, kOpVerify);
-    if (in_bytes + out_bytes + ws + 512 > D.budget)
+    if (in_bytes + out_bytes + ws + 512 > budget)
         return fail(kErrBudget, "verify batch exceeds the device-memory budget%s");
-    rc = D.ensure(in_bytes + out_bytes + ws + 512, in_bytes);
+    int rc = L.ensure(in_bytes + out_bytes + ws + 512, in_bytes);
     if (rc) return rc;
-    uint8_t* h = D.hbuf;
+    uint8_t* h = L.hbuf;
     uint8_t* p = h;
     copy_items(p, o->pk_bytes, pk, pk_stride, o->pk_bytes, num);
     p += al16((uint64_t)num * o->pk_bytes);
@@ -387,8 +490,10 @@ static int verify_host(const SetOps* o, uint32_t num, const uint8_t* pk, uint64_
     memcpy(p, mp.off.data(), 8ull * num);
     p += al16(8ull * num);
     memcpy(p, mlen, 4ull * num);
-    uint8_t* d = (uint8_t*)D.dbuf;
-    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, D.stream));
+    uint8_t* d = (uint8_t*)L.dbuf;
+    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, L.stream));
+    rc = wait_prev(L, prev);
+    if (rc) return rc;
     DevBatch b{};
     b.n = num;
     b.pk = d;
@@ -405,35 +510,54 @@ static int verify_host(const SetOps* o, uint32_t num, const uint8_t* pk, uint64_
     int32_t* dst = (int32_t*)(d + in_bytes);
     b.status = dst;
     b.ws = (uint8_t*)dst + out_bytes;
-    rc = o->verify(b, D.stream);
+    rc = o->verify(b, L.stream);
     if (rc) return fail(rc, "kernel launch failed%s");
-    HIP_OK(hipMemcpyAsync(h, dst, 4ull * num, hipMemcpyDeviceToHost, D.stream));
-    HIP_OK(hipStreamSynchronize(D.stream));
-    memcpy(status, h, 4ull * num);
+    rc = mark_done(L);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(h, dst, 4ull * num, hipMemcpyDeviceToHost, L.stream));
+    job.lane = &L;
+    job.o = o;
+    job.op = kOpVerify;
+    job.num = num;
+    job.out = reinterpret_cast<uint8_t*>(status);
+    return kOk;
+}
+
+// status[i] per item; kErrVerify if any item failed
+static int verify_host(const SetOps* o, uint32_t num, const uint8_t* pk, uint64_t pk_stride,
+                       const uint8_t* sig, uint64_t sig_stride, const uint8_t* msg,
+                       const uint64_t* moff, const uint32_t* mlen, int32_t* status,
+                       int slices = 1) {
+    if (num == 0) return kOk;
+    if (!pk || !sig || !mlen || !moff || !status) return fail(kErrArg, "null argument%s");
+    const int rc = run_slices(num, slices, [&](Lane& L, const Lane* prev, uint32_t a,
+                                               uint32_t cnt, size_t budget, Job& job) {
+        return verify_enqueue(o, L, prev, budget, cnt, pk + a * pk_stride, pk_stride,
+                              sig + a * sig_stride, sig_stride, msg, moff + a, mlen + a,
+                              status + a, job);
+    });
+    if (rc) return rc;
     for (uint32_t i = 0; i < num; i++)
         if (status[i] != 0) return kErrVerify;
     return kOk;
 }
 
-static int keygen_host(const SetOps* o, uint32_t num, const uint8_t* seeds, uint64_t seed_stride,
-                       uint8_t* pk, uint8_t* sk) {
-    if (num == 0) return kOk;
-    if (!seeds || !pk || !sk) return fail(kErrArg, "null argument%s");
-    Device& D = device();
-    std::lock_guard<std::mutex> lk(D.mu);
-    int rc = D.init();
-    if (rc) return rc;
+static int keygen_enqueue(const SetOps* o, Lane& L, const Lane* prev, size_t budget,
+                          uint32_t num, const uint8_t* seeds, uint64_t seed_stride, uint8_t* pk,
+                          uint8_t* sk, Job& job) {
     const uint64_t sb = 3ull * o->n;
     const uint64_t in_bytes = al16(num * sb);
     const uint64_t out_bytes = al16((uint64_t)num * o->pk_bytes) + al16((uint64_t)num * o->sk_bytes);
     const uint64_t ws = o->workspace(num, kOpKeygen);
-    if (in_bytes + out_bytes + ws + 512 > D.budget)
+    if (in_bytes + out_bytes + ws + 512 > budget)
         return fail(kErrBudget, "keygen batch exceeds the device-memory budget%s");
-    rc = D.ensure(in_bytes + out_bytes + ws + 512, in_bytes > out_bytes ? in_bytes : out_bytes);
+    int rc = L.ensure(in_bytes + out_bytes + ws + 512, in_bytes > out_bytes ? in_bytes : out_bytes);
     if (rc) return rc;
-    for (uint32_t i = 0; i < num; i++) memcpy(D.hbuf + i * sb, seeds + i * seed_stride, sb);
-    uint8_t* d = (uint8_t*)D.dbuf;
-    HIP_OK(hipMemcpyAsync(d, D.hbuf, num * sb, hipMemcpyHostToDevice, D.stream));
+    for (uint32_t i = 0; i < num; i++) memcpy(L.hbuf + i * sb, seeds + i * seed_stride, sb);
+    uint8_t* d = (uint8_t*)L.dbuf;
+    HIP_OK(hipMemcpyAsync(d, L.hbuf, num * sb, hipMemcpyHostToDevice, L.stream));
+    rc = wait_prev(L, prev);
+    if (rc) return rc;
     DevBatch b{};
     b.n = num;
     b.seed = d;
@@ -441,13 +565,38 @@ static int keygen_host(const SetOps* o, uint32_t num, const uint8_t* seeds, uint
     b.pk_out = d + in_bytes;
     b.sk_out = b.pk_out + al16((uint64_t)num * o->pk_bytes);
     b.ws = d + in_bytes + out_bytes;
-    rc = o->keygen(b, D.stream);
+    rc = o->keygen(b, L.stream);
     if (rc) return fail(rc, "kernel launch failed%s");
-    HIP_OK(hipMemcpyAsync(D.hbuf, b.pk_out, out_bytes, hipMemcpyDeviceToHost, D.stream));
-    HIP_OK(hipStreamSynchronize(D.stream));
-    memcpy(pk, D.hbuf, (uint64_t)num * o->pk_bytes);
-    memcpy(sk, D.hbuf + al16((uint64_t)num * o->pk_bytes), (uint64_t)num * o->sk_bytes);
+    rc = mark_done(L);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(L.hbuf, b.pk_out, out_bytes, hipMemcpyDeviceToHost, L.stream));
+    job.lane = &L;
+    job.o = o;
+    job.op = kOpKeygen;
+    job.num = num;
+    job.out = pk;
+    job.out2 = sk;
     return kOk;
+}
+
+static int keygen_host(const SetOps* o, uint32_t num, const uint8_t* seeds, uint64_t seed_stride,
+                       uint8_t* pk, uint8_t* sk, int slices = 1) {
+    if (num == 0) return kOk;
+    if (!seeds || !pk || !sk) return fail(kErrArg, "null argument%s");
+    return run_slices(num, slices, [&](Lane& L, const Lane* prev, uint32_t a, uint32_t cnt,
+                                       size_t budget, Job& job) {
+        return keygen_enqueue(o, L, prev, budget, cnt, seeds + a * seed_stride, seed_stride,
+                              pk + (uint64_t)a * o->pk_bytes, sk + (uint64_t)a * o->sk_bytes, job);
+    });
+}
+
+// slices of a multi-stream call: sign and keygen slices of at least 2048
+// items (smaller signing launches leave part of the chip idle), verify
+// slices of at least 1024
+static int ms_slices(int op, uint32_t num) {
+    const uint32_t per = op == kOpVerify ? 1024u : 2048u;
+    const uint32_t s = num / per;
+    return s < 1 ? 1 : s > (uint32_t)kLanes ? kLanes : (int)s;
 }
 
 // draw the per-item optrand the way crypto_sign_signature does
@@ -571,6 +720,7 @@ int crypto_sign_keypair(unsigned char* pk, unsigned char* sk) {
 int crypto_sign_signature(uint8_t* sig, size_t* siglen, const uint8_t* m, size_t mlen,
                           const uint8_t* sk) {
     const SetOps* o = get_set(default_pid());
+    if (mlen > UINT32_MAX) return fail(kErrArg, "message longer than 2^32 - 1 bytes%s");
     uint8_t opt[32];
     draw_optrand(opt, 1, o->n);
     const uint64_t off = 0;
@@ -584,6 +734,7 @@ int crypto_sign_verify(const uint8_t* sig, size_t siglen, const uint8_t* m, size
                        const uint8_t* pk) {
     const SetOps* o = get_set(default_pid());
     if (siglen != (size_t)o->sig_bytes) return kErrVerify;  // sign.cu:3065-3067
+    if (mlen > UINT32_MAX) return fail(kErrArg, "message longer than 2^32 - 1 bytes%s");
     const uint64_t off = 0;
     const uint32_t ml = (uint32_t)mlen;
     int32_t st = -1;
@@ -623,51 +774,100 @@ int crypto_sign_open(unsigned char* m, unsigned long long* mlen, const unsigned 
     return kOk;
 }
 
-int face_mdp_crypto_sign_keypair(unsigned char* pk, unsigned char* sk, unsigned int num) {
+}  // extern "C"
+
+// face_mdp_* / face_ms_mdp_* bodies; slices > 1 pipelines the batch over
+// several streams (ms_slices)
+static int mdp_keypair(unsigned char* pk, unsigned char* sk, unsigned int num, bool ms) {
     const SetOps* o = get_set(default_pid());
+    if (num && (!pk || !sk)) return fail(kErrArg, "null argument%s");
     std::vector<uint8_t> seeds(3ull * o->n * num);
     for (unsigned i = 0; i < num; i++) draw_seed(seeds.data() + 3ull * o->n * i, o->n);
-    return keygen_host(o, num, seeds.data(), 3ull * o->n, pk, sk);
+    return keygen_host(o, num, seeds.data(), 3ull * o->n, pk, sk,
+                       ms ? ms_slices(kOpKeygen, num) : 1);
 }
 
-int face_mdp_crypto_sign(unsigned char* sm, unsigned long long* smlen, const unsigned char* m,
-                         unsigned long long mlen, const unsigned char* sk, unsigned int num) {
+// messages at the fixed SPX_MLEN = 32 byte stride (sign.cu:3500-3502), so
+// mlen may not exceed 32; sm records of SPX_BYTES + 32 bytes hold sig || m_i
+static int mdp_sign(unsigned char* sm, unsigned long long* smlen, const unsigned char* m,
+                    unsigned long long mlen, const unsigned char* sk, unsigned int num, bool ms) {
     const SetOps* o = get_set(default_pid());
+    if (mlen > 32) return fail(kErrArg, "mlen exceeds the 32-byte message stride (SPX_MLEN)%s");
+    if (num && (!sm || !smlen || !sk || (mlen && !m))) return fail(kErrArg, "null argument%s");
     const uint64_t stride = (uint64_t)o->sig_bytes + 32;  // SPX_SM_BYTES (params.h:150)
     std::vector<uint64_t> off(num);
     std::vector<uint32_t> ml(num, (uint32_t)mlen);
-    for (unsigned i = 0; i < num; i++) off[i] = 32ull * i;  // SPX_MLEN stride (sign.cu:3500)
+    for (unsigned i = 0; i < num; i++) off[i] = mlen ? 32ull * i : 0;
     std::vector<uint8_t> opt((uint64_t)num * o->n);
     draw_optrand(opt.data(), num, o->n);
-    int rc = sign_host(o, num, sk, o->sk_bytes, m, off.data(), ml.data(), opt.data(), sm, stride);
+    static const uint8_t kNoMsg[1] = {0};
+    int rc = sign_host(o, num, sk, o->sk_bytes, m ? m : kNoMsg, off.data(), ml.data(), opt.data(),
+                       sm, stride, ms ? ms_slices(kOpSign, num) : 1);
     if (rc) return rc;
-    for (unsigned i = 0; i < num; i++) memcpy(sm + i * stride + o->sig_bytes, m + 32ull * i, mlen);
-    *smlen = stride;
+    for (unsigned i = 0; i < num && mlen; i++)
+        memcpy(sm + i * stride + o->sig_bytes, m + 32ull * i, mlen);
+    if (smlen) *smlen = stride;
     return kOk;
 }
 
-int face_mdp_crypto_sign_open(unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
-                              unsigned long long smlen, const unsigned char* pk, unsigned int num) {
+// records at stride SPX_BYTES + 32 with smlen - SPX_BYTES message bytes
+// (at most 32); output slots of 32 bytes
+static int mdp_open(unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
+                    unsigned long long smlen, const unsigned char* pk, unsigned int num, bool ms) {
     const SetOps* o = get_set(default_pid());
     const uint64_t stride = (uint64_t)o->sig_bytes + 32;
+    if (num && (!m || !mlen || !sm || !pk)) return fail(kErrArg, "null argument%s");
     if (smlen < (unsigned long long)o->sig_bytes) {
         memset(m, 0, 32ull * num);
         *mlen = 0;
         return kErrVerify;
     }
+    if (smlen - o->sig_bytes > 32)
+        return fail(kErrArg, "smlen exceeds the SPX_BYTES + 32 record stride%s");
     const uint32_t ml = (uint32_t)(smlen - o->sig_bytes);
     std::vector<uint64_t> off(num);
     std::vector<uint32_t> mls(num, ml);
     for (unsigned i = 0; i < num; i++) off[i] = i * stride + o->sig_bytes;
     std::vector<int32_t> st(num, -1);
-    int rc = verify_host(o, num, pk, o->pk_bytes, sm, stride, sm, off.data(), mls.data(), st.data());
+    int rc = verify_host(o, num, pk, o->pk_bytes, sm, stride, sm, off.data(), mls.data(), st.data(),
+                         ms ? ms_slices(kOpVerify, num) : 1);
     if (rc != kOk && rc != kErrVerify) return rc;
     for (unsigned i = 0; i < num; i++) {
-        if (st[i] == 0) memcpy(m + 32ull * i, sm + i * stride + o->sig_bytes, ml < 32 ? ml : 32);
+        if (st[i] == 0) memcpy(m + 32ull * i, sm + i * stride + o->sig_bytes, ml);
         else memset(m + 32ull * i, 0, 32);
     }
     *mlen = ml;
     return rc;
+}
+
+extern "C" {
+
+int face_mdp_crypto_sign_keypair(unsigned char* pk, unsigned char* sk, unsigned int num) {
+    return mdp_keypair(pk, sk, num, false);
+}
+
+int face_mdp_crypto_sign(unsigned char* sm, unsigned long long* smlen, const unsigned char* m,
+                         unsigned long long mlen, const unsigned char* sk, unsigned int num) {
+    return mdp_sign(sm, smlen, m, mlen, sk, num, false);
+}
+
+int face_mdp_crypto_sign_open(unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
+                              unsigned long long smlen, const unsigned char* pk, unsigned int num) {
+    return mdp_open(m, mlen, sm, smlen, pk, num, false);
+}
+
+int face_ms_mdp_crypto_sign_keypair(unsigned char* pk, unsigned char* sk, unsigned int num) {
+    return mdp_keypair(pk, sk, num, true);
+}
+
+int face_ms_mdp_crypto_sign(unsigned char* sm, unsigned long long* smlen, unsigned char* m,
+                            unsigned long long mlen, unsigned char* sk, unsigned int num) {
+    return mdp_sign(sm, smlen, m, mlen, sk, num, true);
+}
+
+int face_ms_mdp_crypto_sign_open(unsigned char* m, unsigned long long* mlen, unsigned char* sm,
+                                 unsigned long long smlen, unsigned char* pk, unsigned int num) {
+    return mdp_open(m, mlen, sm, smlen, pk, num, true);
 }
 
 int spx_batch_keygen(int pid, uint32_t num, const uint8_t* seeds, uint8_t* pk, uint8_t* sk) {
@@ -709,7 +909,7 @@ static hipStream_t pick_stream(void* stream) {
     Device& D = device();
     std::lock_guard<std::mutex> lk(D.mu);
     D.init();
-    return D.stream;
+    return D.lane[0].stream;
 }
 
 int cuspx_sign_device(int pid, uint32_t num, const uint8_t* sk, uint32_t sk_stride,

@@ -61,6 +61,9 @@ struct SetOps {
 
 const SetOps* get_set(int pid);
 
+// sets the text cuspx_last_error() reports on this thread; returns code
+int set_error(int code, const char* msg);
+
 // lanes chosen by the tuning table: per WOTS leaf for a k_ht launch over
 // `trees` subtrees (op kOpSign / kOpKeygen), per request for a k_vht launch
 // over `trees` requests (kOpVerify); 0 = no entry

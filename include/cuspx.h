@@ -73,7 +73,8 @@ unsigned long long crypto_sign_seedbytes(void);
 int crypto_sign_seed_keypair(unsigned char *pk, unsigned char *sk, const unsigned char *seed);
 /* replaces api.h:48 / sign.cu:342 (draws 3n DRBG bytes) */
 int crypto_sign_keypair(unsigned char *pk, unsigned char *sk);
-/* replaces api.h:74 / sign.cu:1924 (draws n DRBG bytes of optrand) */
+/* replaces api.h:74 / sign.cu:1924 (draws n DRBG bytes of optrand); messages
+ * of 2^32 bytes or more are refused (-4), here and in the calls below */
 int crypto_sign_signature(uint8_t *sig, size_t *siglen, const uint8_t *m, size_t mlen,
                           const uint8_t *sk);
 /* replaces api.h:80 / sign.cu:3048 */
@@ -93,16 +94,30 @@ int crypto_sign_open(unsigned char *m, unsigned long long *mlen, const unsigned 
  * else 3n DRBG bytes per key in order (the reference draws curand). */
 int face_mdp_crypto_sign_keypair(unsigned char *pk, unsigned char *sk, unsigned int num);
 /* replaces api.h:107 face_mdp_crypto_sign / sign.cu:5098.  Message i is
- * m + 32*i (SPX_MLEN stride) of mlen bytes; sm record i at stride
- * SPX_BYTES+32 holds sig || m_i; *smlen = SPX_BYTES+32. */
+ * m + 32*i (SPX_MLEN stride) of mlen <= 32 bytes (-4 otherwise); sm record i
+ * at stride SPX_BYTES+32 holds sig || m_i; *smlen = SPX_BYTES+32. */
 int face_mdp_crypto_sign(unsigned char *sm, unsigned long long *smlen, const unsigned char *m,
                          unsigned long long mlen, const unsigned char *sk, unsigned int num);
 /* replaces api.h:142 face_mdp_crypto_sign_open / sign.cu:6856.  Records at
- * stride SPX_BYTES+32, message length smlen-SPX_BYTES, output slots of 32
- * bytes.  Returns 0 only if every item verifies (the reference reports
- * item 0 only, sign.cu:6154); failed items get a zeroed slot. */
+ * stride SPX_BYTES+32, message length smlen-SPX_BYTES (at most 32, else -4),
+ * output slots of 32 bytes.  Returns 0 only if every item verifies (the
+ * reference reports item 0 only, sign.cu:6154); failed items get a zeroed
+ * slot. */
 int face_mdp_crypto_sign_open(unsigned char *m, unsigned long long *mlen, const unsigned char *sm,
                               unsigned long long smlen, const unsigned char *pk, unsigned int num);
+/* Multi-stream forms: replace api.h:61 face_ms_mdp_crypto_sign_keypair /
+ * sign.cu:1400, api.h:112 face_ms_mdp_crypto_sign / sign.cu:5305 and
+ * api.h:151 face_ms_mdp_crypto_sign_open / sign.cu:7080 (the reference's
+ * non-const pointers).  Same arguments, outputs and DRBG draws as the
+ * face_mdp_* calls above; the batch is cut into up to 4 slices on separate
+ * streams whose kernels run in order while one slice's PCIe copies and host
+ * staging overlap the next slice's kernels (slices of >= 2048 items for
+ * keygen / sign, >= 1024 for verify; smaller batches run as one slice). */
+int face_ms_mdp_crypto_sign_keypair(unsigned char *pk, unsigned char *sk, unsigned int num);
+int face_ms_mdp_crypto_sign(unsigned char *sm, unsigned long long *smlen, unsigned char *m,
+                            unsigned long long mlen, unsigned char *sk, unsigned int num);
+int face_ms_mdp_crypto_sign_open(unsigned char *m, unsigned long long *mlen, unsigned char *sm,
+                                 unsigned long long smlen, unsigned char *pk, unsigned int num);
 
 /* ------------------------- runtime-parameterised batches with per-item status */
 
@@ -204,16 +219,24 @@ typedef struct {
 
 typedef struct spx_server spx_server;
 int spx_server_create(const spx_server_config *cfg, spx_server **out);
-/* sk may be NULL for verify-only keys; key_id = pid * 65536 + row */
+/* sk may be NULL for a verify-only key (sign requests on it are refused with
+ * -4); key_id = pid * 65536 + row.  At most max_keys_per_set keys per set
+ * (-5 beyond); keys stay resident until spx_server_destroy. */
 int spx_server_add_key(spx_server *s, int pid, const uint8_t *sk, const uint8_t *pk,
                        int32_t *key_id);
 /* op 0 = sign (aux = optrand[n] or NULL to draw it like crypto_sign_signature),
- * op 1 = verify (aux = signature).  -5 if the request can never fit the budget */
+ * op 1 = verify (aux = signature).  -5 if the request can never fit one
+ * slot's share of the budget, or the queue already holds a budget of bytes */
 int spx_server_submit(spx_server *s, int op, int32_t key_id, const uint8_t *msg, uint32_t mlen,
                       const uint8_t *aux, uint64_t *ticket);
-/* 0 done (status: 0 ok / -1 invalid signature), 1 timeout, <0 error */
+/* 0 done (status: 0 ok / -1 invalid signature), 1 timeout (the request stays
+ * live: wait again or cancel), <0 error (-4: unknown or cancelled ticket) */
 int spx_server_wait(spx_server *s, uint64_t ticket, int timeout_ms, uint8_t *sig_out,
                     int32_t *status);
+/* 0: the request was still queued; it leaves the queue (freeing its bytes)
+ * and completes with -4, waking any thread waiting on it.  1: it was already
+ * launched (the GPU work is not stopped; the result is discarded) or done.
+ * -4: unknown ticket. */
 int spx_server_cancel(spx_server *s, uint64_t ticket);
 int spx_server_stats(spx_server *s, spx_server_statistics *st);
 int spx_server_reset_stats(spx_server *s);
