@@ -37,7 +37,11 @@ PARAM_NAMES = [
 
 
 class CuspxError(RuntimeError):
-    pass
+    """A libcuspx call failed; ``code`` is its negative status."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class ParamInfo(ctypes.Structure):
@@ -80,6 +84,9 @@ def lib():
                                            ctypes.c_ulonglong, vp, ctypes.c_uint]
         L.face_mdp_crypto_sign_open.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), vp,
                                                 ctypes.c_ulonglong, vp, ctypes.c_uint]
+        L.face_ms_mdp_crypto_sign_keypair.argtypes = L.face_mdp_crypto_sign_keypair.argtypes
+        L.face_ms_mdp_crypto_sign.argtypes = L.face_mdp_crypto_sign.argtypes
+        L.face_ms_mdp_crypto_sign_open.argtypes = L.face_mdp_crypto_sign_open.argtypes
         L.spx_batch_keygen.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, vp, vp]
         L.spx_batch_sign.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
         L.spx_batch_verify.argtypes = [ctypes.c_int, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]
@@ -113,7 +120,7 @@ def lib():
 
 def _check(rc, what):
     if rc not in (OK, ERR_VERIFY):
-        raise CuspxError("%s failed (%d): %s" % (what, rc, lib().cuspx_last_error().decode()))
+        raise CuspxError("%s failed (%d): %s" % (what, rc, lib().cuspx_last_error().decode()), rc)
     return rc
 
 
@@ -150,7 +157,7 @@ def set_split(pid, op, lanes_per_leaf):
     verification (batches up to 256 requests)."""
     rc = lib().cuspx_set_split(pid, op, lanes_per_leaf)
     if rc != OK:
-        raise CuspxError("cuspx_set_split failed (%d)" % rc)
+        raise CuspxError("cuspx_set_split failed (%d)" % rc, rc)
 
 
 def get_split(pid, op, num):
@@ -235,15 +242,17 @@ def crypto_sign_verify(sig, m, pk):
 
 # ------------------------------------------------- face_mdp (sign.cu)
 
-def face_mdp_crypto_sign_keypair(num):
+def face_mdp_crypto_sign_keypair(num, ms=False):
+    """ms: the multi-stream form face_ms_mdp_crypto_sign_keypair (same bytes)."""
     L = lib()
     pk = np.zeros((num, L.crypto_sign_publickeybytes()), np.uint8)
     sk = np.zeros((num, L.crypto_sign_secretkeybytes()), np.uint8)
-    _check(L.face_mdp_crypto_sign_keypair(_ptr(pk), _ptr(sk), num), "face_mdp_crypto_sign_keypair")
+    name = "face_ms_mdp_crypto_sign_keypair" if ms else "face_mdp_crypto_sign_keypair"
+    _check(getattr(L, name)(_ptr(pk), _ptr(sk), num), name)
     return pk, sk
 
 
-def face_mdp_crypto_sign(m, mlen, sk, num):
+def face_mdp_crypto_sign(m, mlen, sk, num, ms=False):
     """m: uint8 array of num*32 bytes (SPX_MLEN stride); returns (sm, smlen)."""
     L = lib()
     stride = L.crypto_sign_bytes() + 32
@@ -251,20 +260,21 @@ def face_mdp_crypto_sign(m, mlen, sk, num):
     smlen = ctypes.c_ulonglong()
     m = np.ascontiguousarray(m, np.uint8)
     sk = np.ascontiguousarray(sk, np.uint8)
-    _check(L.face_mdp_crypto_sign(_ptr(sm), ctypes.byref(smlen), _ptr(m), mlen, _ptr(sk), num),
-           "face_mdp_crypto_sign")
+    name = "face_ms_mdp_crypto_sign" if ms else "face_mdp_crypto_sign"
+    _check(getattr(L, name)(_ptr(sm), ctypes.byref(smlen), _ptr(m), mlen, _ptr(sk), num), name)
     return sm, smlen.value
 
 
-def face_mdp_crypto_sign_open(sm, smlen, pk, num):
+def face_mdp_crypto_sign_open(sm, smlen, pk, num, ms=False):
     """Returns (rc, m[num][32], mlen)."""
     L = lib()
     m = np.zeros((num, 32), np.uint8)
     mlen = ctypes.c_ulonglong()
     sm = np.ascontiguousarray(sm, np.uint8)
     pk = np.ascontiguousarray(pk, np.uint8)
-    rc = _check(L.face_mdp_crypto_sign_open(_ptr(m), ctypes.byref(mlen), _ptr(sm), smlen, _ptr(pk),
-                                            num), "face_mdp_crypto_sign_open")
+    name = "face_ms_mdp_crypto_sign_open" if ms else "face_mdp_crypto_sign_open"
+    rc = _check(getattr(L, name)(_ptr(m), ctypes.byref(mlen), _ptr(sm), smlen, _ptr(pk), num),
+                name)
     return rc, m, mlen.value
 
 
@@ -497,7 +507,7 @@ class Server:
         a = None if aux is None else np.frombuffer(bytes(aux), np.uint8)
         rc = lib().spx_server_submit(self.h, op, key, _ptr(m), len(msg), _ptr(a), ctypes.byref(t))
         if rc != OK:
-            raise CuspxError("spx_server_submit failed (%d)" % rc)
+            raise CuspxError("spx_server_submit failed (%d)" % rc, rc)
         return t.value
 
     def submit_sign(self, key, msg, optrand=None):

@@ -55,8 +55,10 @@ struct Request {
     std::vector<uint8_t> aux;  // sign: optrand (n); verify: signature
     std::vector<uint8_t> sig;  // sign result
     int32_t status = 0;
+    int cls = 0;            // message-size class (queue key)
     bool done = false;
     bool cancelled = false;
+    bool launched = false;  // taken from its queue into a batch
     Clock::time_point t_submit, t_launch, t_done;
 };
 using ReqP = std::shared_ptr<Request>;
@@ -65,6 +67,7 @@ struct KeyTable {
     uint8_t* d_sk = nullptr;
     uint8_t* d_pk = nullptr;
     uint32_t* d_mid = nullptr;
+    std::vector<bool> has_sk;  // rows added without a secret key verify only
     uint32_t count = 0, cap = 0;
 };
 
@@ -283,10 +286,7 @@ void Server::loop() {
                 ReqP r = best->front();
                 best->pop_front();
                 pending_bytes -= r->msg.size() + r->aux.size();
-                if (r->cancelled) {
-                    live.erase(r->ticket);
-                    continue;
-                }
+                r->launched = true;
                 reqs.push_back(r);
             }
             if (reqs.empty()) continue;
@@ -379,6 +379,7 @@ int spx_server_add_key(spx_server* sv, int pid, const uint8_t* sk, const uint8_t
     }
     if (kt.count >= kt.cap) return kErrBudget;
     const uint32_t row = kt.count;
+    if (kt.has_sk.size() < kt.cap) kt.has_sk.resize(kt.cap, false);
     std::vector<uint8_t> skb(o->sk_bytes, 0);
     if (sk) memcpy(skb.data(), sk, o->sk_bytes);
     if (hipMemcpy(kt.d_sk + (uint64_t)row * o->sk_bytes, skb.data(), o->sk_bytes,
@@ -389,6 +390,7 @@ int spx_server_add_key(spx_server* sv, int pid, const uint8_t* sk, const uint8_t
     int rc = o->midstates(kt.d_pk + (uint64_t)row * o->pk_bytes, o->pk_bytes, kt.d_mid + row * 8,
                           1, s->setup);
     if (rc || hipStreamSynchronize(s->setup) != hipSuccess) return kErrHip;
+    kt.has_sk[row] = sk != nullptr;
     kt.count++;
     *key_id = pid * 65536 + (int32_t)row;
     return kOk;
@@ -424,6 +426,7 @@ int spx_server_submit(spx_server* sv, int op, int32_t key_id, const uint8_t* msg
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->keys.find(pid);
         if (it == s->keys.end() || row >= it->second.count) return kErrArg;
+        if (op == 0 && !it->second.has_sk[row]) return kErrArg;  // verify-only key
         // one request must fit a slot; the queue holds at most one budget of bytes
         const uint64_t need = s->item_bytes(o, op) + al16(mlen) + 8192;
         const uint64_t hneed = (uint64_t)o->sig_bytes + o->n + al16(mlen) + 8192;
@@ -434,6 +437,7 @@ int spx_server_submit(spx_server* sv, int op, int32_t key_id, const uint8_t* msg
         r->t_submit = Clock::now();
         s->pending_bytes += mlen + r->aux.size();
         const int cls = mlen <= 2048 ? 0 : (mlen <= 16384 ? 1 : 2);
+        r->cls = cls;
         s->queues[std::make_tuple(pid, op, cls)].push_back(r);
         s->live[r->ticket] = r;
         *ticket = r->ticket;
@@ -463,12 +467,35 @@ int spx_server_wait(spx_server* sv, uint64_t ticket, int timeout_ms, uint8_t* si
 int spx_server_cancel(spx_server* sv, uint64_t ticket) {
     Server* s = (Server*)sv;
     if (!s) return kErrArg;
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->live.find(ticket);
     if (it == s->live.end()) return kErrArg;
-    it->second->cancelled = true;
-    if (it->second->done) s->live.erase(it);
-    return kOk;
+    ReqP r = it->second;
+    r->cancelled = true;
+    if (!r->launched && !r->done) {
+        // still queued: leave the queue now (its bytes count no more) and
+        // complete with -4, waking any thread waiting on it
+        auto q = s->queues.find(std::make_tuple(r->pid, r->op, r->cls));
+        if (q != s->queues.end()) {
+            auto& dq = q->second;
+            for (auto x = dq.begin(); x != dq.end(); ++x)
+                if (x->get() == r.get()) {
+                    dq.erase(x);
+                    break;
+                }
+        }
+        s->pending_bytes -= r->msg.size() + r->aux.size();
+        r->status = kErrArg;
+        r->done = true;
+        r->t_done = Clock::now();
+        s->live.erase(it);
+        lk.unlock();
+        s->cv_done.notify_all();
+        return kOk;
+    }
+    // launched (the GPU work is not stopped; the result is dropped) or done
+    if (r->done) s->live.erase(it);
+    return 1;
 }
 
 static float pct(std::vector<float> v, double q) {

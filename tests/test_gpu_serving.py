@@ -112,3 +112,79 @@ def test_server_admission_and_cancel():
             if i == 5:
                 continue
             assert srv.wait(t, key=kid)[0] == 0
+
+
+def test_server_refusals():
+    """A sign request on a key added without a secret key is refused (-4),
+    verifies on it run; a request that can never fit one slot's share of the
+    budget is refused (-5)."""
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(bytes(range(3 * p.n)), np.uint8))
+    sig = C.batch_sign(pid, sk, [b"verify-only"], np.zeros(p.n, np.uint8))
+    with C.Server(hbm_budget_bytes=1 << 30, slots=2) as srv:
+        vk = srv.add_key(pid, None, bytes(pk[0]))
+        with pytest.raises(C.CuspxError) as e:
+            srv.submit_sign(vk, b"m")
+        assert e.value.code == C.ERR_ARG
+        assert srv.wait(srv.submit_verify(vk, b"verify-only", bytes(sig[0])))[0] == 0
+        assert srv.wait(srv.submit_verify(vk, b"verify-Only", bytes(sig[0])))[0] == -1
+    # 8 MiB over 4 slots: a SHA2-128s signing workspace (about 2 MiB) never fits
+    pid_s = C.param_id("sha2-128s")
+    ps = O.params(pid_s)
+    pk_s, sk_s = C.batch_keygen(pid_s, np.frombuffer(bytes(3 * ps.n), np.uint8))
+    with C.Server(hbm_budget_bytes=8 << 20, slots=4) as srv:
+        k = srv.add_key(pid_s, bytes(sk_s[0]), bytes(pk_s[0]))
+        with pytest.raises(C.CuspxError) as e:
+            srv.submit_sign(k, b"m")
+        assert e.value.code == C.ERR_BUDGET
+
+
+def test_server_wait_timeout_and_cancel_before_launch():
+    """One slot runs a long SHAKE256-256f request (a 64 KiB message); a
+    second request queues behind it.  A wait with timeout 0 on the running
+    request returns 1 (TimeoutError) and leaves it live; cancelling the
+    queued request removes it from the queue at once (pending drops to 0)
+    and wakes the thread already waiting on it with -4; the running request
+    then completes normally."""
+    import threading
+    pid_l = C.param_id("shake256-256f")
+    pl = O.params(pid_l)
+    pk_l, sk_l = C.batch_keygen(pid_l, np.frombuffer(bytes(range(3 * pl.n)), np.uint8))
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(bytes(range(3 * p.n)), np.uint8))
+    long_msg = bytes(range(256)) * 256
+    with C.Server(hbm_budget_bytes=1 << 30, slots=1, batch_window_us=0) as srv:
+        kl = srv.add_key(pid_l, bytes(sk_l[0]), bytes(pk_l[0]))
+        ks = srv.add_key(pid, bytes(sk[0]), bytes(pk[0]))
+        import time
+        t_long = srv.submit_sign(kl, long_msg, bytes(pl.n))
+        deadline = time.time() + 5
+        while srv.stats()["pending"] and time.time() < deadline:
+            time.sleep(1e-4)  # the long request has left the queue for the slot
+        t_q = srv.submit_sign(ks, b"queued", bytes(p.n))
+        assert srv.stats()["pending"] == 1
+        with pytest.raises(TimeoutError):
+            srv.wait(t_long, timeout_ms=0)
+        got = {}
+
+        def waiter():
+            try:
+                got["r"] = srv.wait(t_q, timeout_ms=20000)
+            except C.CuspxError as e:
+                got["code"] = e.code
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        assert srv.cancel(t_q) == 0  # still queued behind the long request
+        th.join(timeout=10)
+        assert not th.is_alive() and got.get("code") == C.ERR_ARG
+        assert srv.stats()["pending"] == 0
+        st, sig = srv.wait(t_long, key=kl)
+        assert st == 0
+        assert sig == O.signature_with_optrand(pid_l, long_msg, bytes(sk_l[0]), bytes(pl.n))
+        # a finished request cannot be cancelled any more
+        t2 = srv.submit_sign(ks, b"after", bytes(p.n))
+        assert srv.wait(t2, key=ks)[0] == 0
+        assert srv.cancel(t2) == C.ERR_ARG
