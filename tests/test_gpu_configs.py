@@ -1,0 +1,222 @@
+"""The benchmark configurations at their full sizes, bit-exact on the GPU.
+
+Inputs come from cuspx_amd/workloads.py (the same generators bench.py
+times), drawn with the library's DRBG; expected outputs are the CPU
+oracle's, committed as per-item digests in tests/golden/configs.json
+(written by tests/golden/make_configs.py).  Every signature of every batch
+is compared, and every verify status.
+
+* config 2: SHA2-128f, 4096 KAT-protocol keypairs (DRBG mode), one 32-byte
+  message each, device-resident keygen / sign / verify launches (the bench
+  path).  Reference semantics: sign.cu:1924-1994, 3048-3121.
+* config 3: 8192 SHAKE256-128f / -192f requests, messages 1 B .. 4 KiB, one
+  mixed-set graph launch.
+* config 4: SHA2-128s batches of 1, 64 and 1024 captured as hipGraphs and
+  replayed, with eager launches between the replays.
+* config 5: the serving loop with 256 resident keys over SHA2-128f /
+  SHA2-128s / SHAKE256-256f, messages 16 B .. 64 KiB, 80% sign / 20%
+  verify (a quarter of the verifies tampered), arrivals at a fixed rate.
+"""
+import hashlib
+import json
+import os
+import random
+import time
+
+import numpy as np
+import pytest
+
+import cuspx_amd as C
+from cuspx_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = json.load(open(os.path.join(HERE, "golden", "configs.json")))
+
+
+class LibDrbg:
+    randombytes_init = staticmethod(C.randombytes_init)
+    randombytes = staticmethod(C.randombytes)
+
+
+def _digests(hexstr):
+    b = bytes.fromhex(hexstr)
+    return [b[i:i + 8] for i in range(0, len(b), 8)]
+
+
+def _compare(name, sigs, want):
+    assert len(sigs) == len(want), name
+    bad = [i for i, s in enumerate(sigs) if W.digest(s) != want[i]]
+    assert bad == [], "%s: %d of %d signatures differ from the oracle, first %s" % (
+        name, len(bad), len(sigs), bad[:8])
+
+
+def _pk_digest(pks):
+    return hashlib.sha256(b"".join(bytes(x) for x in pks)).hexdigest()
+
+
+def test_config2_sha2_128f_kat_batch_4096():
+    g = G["config2"]
+    pid = C.param_id("sha2-128f")
+    p = C.params(pid)
+    B = g["count"]
+    kseeds, msgs, opts = W.kat_batch(LibDrbg, p.n, B)
+    D = C.DeviceBuffer
+    d_seed = D.from_array(np.frombuffer(b"".join(kseeds), np.uint8))
+    d_opt = D.from_array(np.frombuffer(b"".join(opts), np.uint8))
+    d_msg = D.from_array(np.frombuffer(b"".join(msgs), np.uint8))
+    d_moff = D.from_array(np.arange(B, dtype=np.uint64) * 32)
+    d_mlen = D.from_array(np.full(B, 32, np.uint32))
+    d_pk, d_sk, d_sig = D(B * p.pk_bytes), D(B * p.sk_bytes), D(B * p.sig_bytes)
+    d_st = D(4 * B)
+    ws = D(max(C.workspace_bytes(pid, B, op) for op in (0, 1, 2)))
+    st = C.Stream()
+    print("config2: keygen + sign of %d on the device" % B, flush=True)
+    C.keygen_device(pid, B, d_seed.ptr, 3 * p.n, d_pk.ptr, d_sk.ptr, ws.ptr, st.handle)
+    C.sign_device(pid, B, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr, d_opt.ptr,
+                  d_sig.ptr, p.sig_bytes, ws.ptr, st.handle)
+    st.sync()
+    pk = d_pk.download().reshape(B, p.pk_bytes)
+    assert _pk_digest(pk) == g["pk_sha256"]
+    sig = d_sig.download().reshape(B, p.sig_bytes)
+    _compare("config2", [bytes(s) for s in sig], _digests(g["sig"]))
+    d_st.fill(0x7F)
+    C.verify_device(pid, B, d_pk.ptr, p.pk_bytes, d_sig.ptr, p.sig_bytes, d_msg.ptr, d_moff.ptr,
+                    d_mlen.ptr, d_st.ptr, ws.ptr, st.handle)
+    st.sync()
+    assert np.count_nonzero(d_st.download(np.int32)) == 0
+    # one flipped byte in three signatures: exactly those fail
+    bad = [5, 2047, 4095]
+    for i in bad:
+        sig[i, p.n + 1000 + i % 7] ^= 0x20
+    d_sig.upload(sig)
+    C.verify_device(pid, B, d_pk.ptr, p.pk_bytes, d_sig.ptr, p.sig_bytes, d_msg.ptr, d_moff.ptr,
+                    d_mlen.ptr, d_st.ptr, ws.ptr, st.handle)
+    st.sync()
+    assert list(np.nonzero(d_st.download(np.int32))[0]) == bad
+
+
+def _keygen_items(names_seeds):
+    """Keys on the GPU for [(set name, seed)], in order (one batch per set)."""
+    out = [None] * len(names_seeds)
+    by = {}
+    for i, (name, seed) in enumerate(names_seeds):
+        by.setdefault(name, []).append(i)
+    for name, idx in by.items():
+        pid = C.param_id(name)
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(names_seeds[i][1] for i in idx),
+                                                   np.uint8))
+        for j, i in enumerate(idx):
+            out[i] = (bytes(pk[j]), bytes(sk[j]))
+    return out
+
+
+def test_config3_mixed_shake_8192_one_graph():
+    g = G["config3"]
+    items = W.mixed_shake_batch(LibDrbg, g["count"])
+    print("config3: keygen of %d" % len(items), flush=True)
+    keys = _keygen_items([(name, seed) for name, seed, _, _ in items])
+    assert _pk_digest(pk for pk, _ in keys) == g["pk_sha256"]
+    batch = [(C.param_id(name), sk, m, opt) for (name, _, m, opt), (_, sk) in zip(items, keys)]
+    print("config3: mixed graph of %d" % len(batch), flush=True)
+    mb = C.MixedSignBatch(batch)
+    st = C.Stream()
+    mb.launch(st)
+    st.sync()
+    sigs = mb.signatures()
+    _compare("config3", sigs, _digests(g["sig"]))
+    # every signature verifies (one batch per set)
+    for name in g["sets"]:
+        pid = C.param_id(name)
+        idx = [i for i, it in enumerate(items) if it[0] == name]
+        p = C.params(pid)
+        pk = np.frombuffer(b"".join(keys[i][0] for i in idx), np.uint8).reshape(-1, p.pk_bytes)
+        s = np.frombuffer(b"".join(sigs[i] for i in idx), np.uint8).reshape(-1, p.sig_bytes)
+        stt = C.batch_verify(pid, pk, s, [items[i][2] for i in idx])
+        assert np.count_nonzero(stt) == 0, name
+
+
+def test_config4_sha2_128s_graph_replays():
+    g = G["config4"]["batches"]
+    pid = C.param_id("sha2-128s")
+    p = C.params(pid)
+    D = C.DeviceBuffer
+    for B, (seeds, msgs, opts) in W.sha2_128s_batches().items():
+        print("config4: batch %d" % B, flush=True)
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+        assert _pk_digest(pk) == g[str(B)]["pk_sha256"]
+        d_sk = D.from_array(sk)
+        d_msg = D.from_array(np.frombuffer(b"".join(msgs), np.uint8))
+        d_opt = D.from_array(np.frombuffer(b"".join(opts), np.uint8))
+        d_moff = D.from_array(np.arange(B, dtype=np.uint64) * 32)
+        d_mlen = D.from_array(np.full(B, 32, np.uint32))
+        d_sig = D(B * p.sig_bytes)
+        ws = D(C.workspace_bytes(pid, B, C.OP_SIGN))
+        gr = C.graph_sign(pid, B, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr,
+                          d_opt.ptr, d_sig.ptr, p.sig_bytes, ws.ptr)
+        want = _digests(g[str(B)]["sig"])
+        st = C.Stream()
+        for rep in range(2):
+            d_sig.fill(0)
+            gr.launch(st)
+            st.sync()
+            sig = d_sig.download().reshape(B, p.sig_bytes)
+            _compare("config4 batch %d replay %d" % (B, rep), [bytes(s) for s in sig], want)
+            # eager launches (the host-buffer verify path) between the replays
+            stt = C.batch_verify(pid, pk, sig, msgs)
+            assert np.count_nonzero(stt) == 0
+        gr.close()
+
+
+def test_config5_serving_mix_bit_exact():
+    g = G["config5"]
+    kdef = W.serving_keys(LibDrbg)
+    print("config5: keygen of %d serving keys" % len(kdef), flush=True)
+    keys = _keygen_items(kdef)
+    assert _pk_digest(pk for pk, _ in keys) == g["pk_sha256"]
+    sources, reqs = W.serving_requests(g["requests"])
+    n_of = {name: C.params(C.param_id(name)).n for name in W.SERVE_SETS}
+    with C.Server(hbm_budget_bytes=8 << 30, slots=4, batch_window_us=1000) as srv:
+        kids = [srv.add_key(C.param_id(name), sk, pk)
+                for (name, _), (pk, sk) in zip(kdef, keys)]
+        # the verify sources, signed through the server
+        print("config5: %d source signatures" % len(sources), flush=True)
+        tk = [srv.submit_sign(kids[k], m, o[:n_of[kdef[k][0]]]) for k, m, o in sources]
+        src_sigs = []
+        for t, (k, _, _) in zip(tk, sources):
+            s, sig = srv.wait(t, key=kids[k])
+            assert s == 0
+            src_sigs.append(sig)
+        _compare("config5 sources", src_sigs, _digests(g["source_sig"]))
+        # the mixed stream at a fixed rate: batches overlap on the device
+        print("config5: stream of %d requests" % len(reqs), flush=True)
+        rng = random.Random(7)
+        tickets = []
+        t_next = time.perf_counter()
+        for r in reqs:
+            t_next += rng.expovariate(3000.0)
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            if r[0] == "sign":
+                _, k, m, o = r
+                tickets.append(("sign", k, srv.submit_sign(kids[k], m, o[:n_of[kdef[k][0]]])))
+            else:
+                _, vi, tamper = r
+                k, m, _ = sources[vi]
+                m2, s2 = W.apply_tamper(tamper, m, src_sigs[vi])
+                tickets.append(("verify", k, srv.submit_verify(kids[k], m2, s2)))
+        signs, status = [], []
+        for op, k, t in tickets:
+            s, sig = srv.wait(t, key=kids[k])
+            if op == "sign":
+                assert s == 0
+                signs.append(sig)
+            else:
+                status.append(s)
+        st = srv.stats()
+    _compare("config5 stream", signs, _digests(g["sign_sig"]))
+    assert status == g["verify_status"]
+    assert st["last_error"] == 0
+    assert st["mean_batch"] > 1.0, "no two requests were batched together"
