@@ -24,23 +24,22 @@ sys.path.insert(0, ROOT)
 # serialise (read by the HIP runtime when it initialises).
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
-# MI355X VALU issue rates (cuspx_amd/valu_rate, profiles/valu_issue_rates.md): the
-# simple ops (v_xor/v_add/v_lshrrev ..._e32, v_bitop3_b32) issue a wave64 instruction
-# every 2 cycles per SIMD, i.e. 32 lanes/clk -- the rate behind the published
-# 157.3 TFLOP/s fp32 vector peak; the other 3-operand ops (v_alignbit, v_add3,
-# v_perm, 64-bit shifts) every 4 cycles.
+# MI355X VALU issue rates (measured by an issue-rate probe, profiles/valu_issue_rates.md):
+# the simple ops (v_xor/v_add/v_lshrrev ..._e32, v_bitop3_b32) issue a wave64
+# instruction every 2 cycles per SIMD, i.e. 32 lanes/clk -- the rate behind the
+# published 157.3 TFLOP/s fp32 vector peak; the other 3-operand ops (v_alignbit,
+# v_add3, v_perm, 64-bit shifts) every 4 cycles.
 CLK_HZ = 2.4e9
 SIMDS = 256 * 4
 PEAK_VALU_T = SIMDS * 32 * CLK_HZ / 1e12    # 78.64 T lane-instr/s (fast class)
 PEAK_VOP3_T = SIMDS * 16 * CLK_HZ / 1e12    # 39.32 T lane-instr/s (4-cycle class)
-# VALU lane-instructions per unit of the shipped k_ht: SQ_INSTS_VALU x 64 over the
-# units of one batch-4096 launch (profiles/k_sign_fused_sha2_128f_b4096.md,
-# profiles/k_sign_fused_shake256_128f_b4096.md)
+# VALU lane-instructions per unit when profiles/pmc_counters.json has no
+# SQ_INSTS_VALU count for the kernel and batch (that count takes precedence)
 SHA256_OPS = 1237
 KECCAK_OPS = 4163
 SHA256_OPS_SURVEY = 1450                   # the per-compression figure the plan priced peak at
 # Issue rate of independent instruction blocks in the proportions of the unit's
-# loop body (wave-instructions per SIMD per ns, cuspx_amd/valu_rate "mix" rows):
+# loop body (wave-instructions per SIMD per ns, "mix" rows of profiles/valu_issue_rates.md):
 # SHA-256 F = 717 four-cycle (508 alignbit, 205 add3, 4 perm) : 506 two-cycle
 # (312 bitop3, 194 VOP2) ~ 7:5; Keccak-f round of a WOTS chain step (bit-interleaved)
 # = 52 alignbit : 122 bitop3/xor ~ 3:7.
@@ -77,17 +76,76 @@ def fors_work(p, batch):
     return batch * p.k * ((1 << p.a) * 2 + ((1 << p.a) - 1) * hblk)
 
 
-def cpu_baseline(pid, seconds, procs):
-    """Time the oracle (single-threaded C port of the reference host path) in
-    `procs` forked processes; returns aggregate signs/s and verifies/s."""
-    import multiprocessing as mp
+def host_cpus():
+    """(CPU model, logical CPUs this process may use, one of them per
+    physical core) from /proc/cpuinfo and the sysfs topology."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = sorted(os.sched_getaffinity(0))
+    seen, phys = set(), []
+    for c in allowed:
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            key = (open(base + "physical_package_id").read().strip(),
+                   open(base + "core_id").read().strip())
+        except OSError:
+            key = ("?", str(c))
+        if key not in seen:
+            seen.add(key)
+            phys.append(c)
+    return model, allowed, phys
 
-    def work(q, k):
+
+def cpu_baseline(pid, seconds, max_procs):
+    """The oracle (single-threaded C restatement of the reference host path)
+    timed on the host cores, as BASELINE.md plans it:
+
+    * one core: one process pinned to one physical core, 2 warm-up then 10
+      timed signatures (32-byte messages) and 10 verifications; medians on
+      CLOCK_MONOTONIC and CLOCK_PROCESS_CPUTIME_ID;
+    * all cores: one process per physical core (at most max_procs), each
+      pinned to its core, signing for 0.8 x `seconds` then verifying for
+      0.2 x `seconds`; aggregate rate = sum of the per-process rates."""
+    import multiprocessing as mp
+    import statistics
+
+    def setup(k):
         from oracle import oracle as O
         import random
         p = O.params(pid)
         rng = random.Random(k)
         pk, sk = O.seed_keypair(pid, rng.randbytes(3 * p.n))
+        return O, p, rng, pk, sk
+
+    def single(q, cpu):
+        os.sched_setaffinity(0, {cpu})
+        O, p, rng, pk, sk = setup(0)
+        mono, proc, vmono, vproc, sigs = [], [], [], [], []
+        for it in range(12):
+            m, opt = rng.randbytes(32), rng.randbytes(p.n)
+            a, b = time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.process_time_ns()
+            s = O.signature_with_optrand(pid, m, sk, opt)
+            a2, b2 = time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.process_time_ns()
+            assert O.verify(pid, s, m, pk) == 0
+            a3, b3 = time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.process_time_ns()
+            if it >= 2:
+                mono.append(a2 - a)
+                proc.append(b2 - b)
+                vmono.append(a3 - a2)
+                vproc.append(b3 - b2)
+        md = lambda x: statistics.median(x) / 1e6
+        q.put({"sign_ms_monotonic": md(mono), "sign_ms_process_cpu": md(proc),
+               "verify_ms_monotonic": md(vmono), "verify_ms_process_cpu": md(vproc)})
+
+    def work(q, k, cpu):
+        os.sched_setaffinity(0, {cpu})
+        O, p, rng, pk, sk = setup(k)
         sigs, n_sign, t0 = [], 0, time.perf_counter()
         while time.perf_counter() - t0 < 0.8 * seconds or n_sign == 0:
             m = rng.randbytes(32)
@@ -101,31 +159,39 @@ def cpu_baseline(pid, seconds, procs):
             n_ver += 1
         q.put((n_sign / t_sign, n_ver / (time.perf_counter() - t0)))
 
+    model, allowed, phys = host_cpus()
+    cores = phys[:max_procs]
     ctx = mp.get_context("fork")
     q = ctx.Queue()
-    ps = [ctx.Process(target=work, args=(q, k)) for k in range(procs)]
+    p1 = ctx.Process(target=single, args=(q, cores[0]))
+    p1.start()
+    one = q.get()
+    p1.join()
+    ps = [ctx.Process(target=work, args=(q, k, c)) for k, c in enumerate(cores)]
     for p in ps:
         p.start()
     res = [q.get() for _ in ps]
     for p in ps:
         p.join()
-    return sum(r[0] for r in res), sum(r[1] for r in res), res[0][0]
+    return {"signs_per_s": sum(r[0] for r in res), "verifies_per_s": sum(r[1] for r in res),
+            "processes": len(cores), "single_core": one, "cpu_model": model,
+            "physical_cores_visible": len(phys), "logical_cpus_visible": len(allowed)}
+
+
+class LibDrbg:
+    """The library's host DRBG as the workload generators' draw source."""
+
+    def __init__(self, C):
+        self.randombytes_init = C.randombytes_init
+        self.randombytes = C.randombytes
 
 
 def make_inputs(C, pid, batch):
-    """KAT-protocol inputs: per-key 3n-byte seeds, optrand, 32-byte messages."""
+    """KAT-protocol inputs (workloads.kat_batch): per-key 3n-byte seeds,
+    optrand, 32-byte messages."""
     import numpy as np
-    p = C.params(pid)
-    C.randombytes_init(bytes(range(48)))
-    seeds48, msgs = [], []
-    for _ in range(batch):
-        seeds48.append(C.randombytes(48))
-        msgs.append(C.randombytes(32))
-    kseeds, opts = [], []
-    for s in seeds48:
-        C.randombytes_init(s)
-        kseeds.append(C.randombytes(3 * p.n))   # crypto_sign_keypair draw (sign.cu:345)
-        opts.append(C.randombytes(p.n))         # crypto_sign optrand draw (sign.cu:1950)
+    from cuspx_amd import workloads as W
+    kseeds, msgs, opts = W.kat_batch(LibDrbg(C), C.params(pid).n, batch)
     to = lambda xs: np.frombuffer(b"".join(xs), np.uint8).copy()
     return to(kseeds), to(opts), to(msgs)
 
@@ -146,28 +212,37 @@ def host_phase(C, pid, sk, pk, m, num, reps):
     smlen, mlen = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
 
-    def sign():
-        assert L.face_mdp_crypto_sign(ptr(sm), ctypes.byref(smlen), ptr(m), 32, ptr(sk), num) == 0
+    out = {}
+    for pre in ("face_mdp", "face_ms_mdp"):
+        fs, fo = getattr(L, pre + "_crypto_sign"), getattr(L, pre + "_crypto_sign_open")
 
-    def open_():
-        assert L.face_mdp_crypto_sign_open(ptr(mo), ctypes.byref(mlen), ptr(sm), smlen.value,
-                                           ptr(pk), num) == 0
+        def sign():
+            assert fs(ptr(sm), ctypes.byref(smlen), ptr(m), 32, ptr(sk), num) == 0
 
-    sign()
-    open_()  # grows the library's device arena and pinned staging once
-    ts, tv = [], []
-    for _ in range(reps):
-        t0 = time.perf_counter()
+        def open_():
+            assert fo(ptr(mo), ctypes.byref(mlen), ptr(sm), smlen.value, ptr(pk), num) == 0
+
         sign()
-        ts.append(time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        open_()
-        tv.append(time.perf_counter() - t0)
-    assert bytes(mo.reshape(-1)) == bytes(m)
-    return {"signs_per_s": round(num / min(ts), 1), "verifies_per_s": round(num / min(tv), 1),
-            "sign_call_ms": round(min(ts) * 1e3, 3), "open_call_ms": round(min(tv) * 1e3, 3),
-            "calls": "face_mdp_crypto_sign / face_mdp_crypto_sign_open on pageable host "
-                     "buffers (%d items, best of %d); PCIe-inclusive, never `value`" % (num, reps)}
+        open_()  # grows the library's device arenas and pinned staging once
+        ts, tv = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            sign()
+            ts.append(time.perf_counter() - t0)
+            mo.fill(1)
+            t0 = time.perf_counter()
+            open_()
+            tv.append(time.perf_counter() - t0)
+            assert bytes(mo.reshape(-1)) == bytes(m)
+        key = "" if pre == "face_mdp" else "ms_"
+        out.update({key + "signs_per_s": round(num / min(ts), 1),
+                    key + "verifies_per_s": round(num / min(tv), 1),
+                    key + "sign_call_ms": round(min(ts) * 1e3, 3),
+                    key + "open_call_ms": round(min(tv) * 1e3, 3)})
+    out["calls"] = ("face_mdp_crypto_sign / face_mdp_crypto_sign_open and the multi-stream "
+                    "face_ms_mdp_* (ms_ keys) on pageable host buffers (%d items, best of %d); "
+                    "PCIe-inclusive, never `value`" % (num, reps))
+    return out
 
 
 SERVE_SETS = ["sha2-128f", "sha2-128s", "shake256-256f"]
@@ -183,21 +258,19 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_
     throughput of plain per-request launches on the same mix."""
     import random
     import numpy as np
+    from cuspx_amd import workloads as W
     rng = random.Random(seed)
-    C.randombytes_init(bytes([seed]) * 48)
     keys = []
-    for s_i, name in enumerate(SERVE_SETS):
+    kdef = W.serving_keys(LibDrbg(C), seed)
+    for name in SERVE_SETS:
         pid = C.param_id(name)
-        p = C.params(pid)
-        cnt = 86 if s_i < 2 else 84
-        pk, sk = C.batch_keygen(pid, np.frombuffer(C.randombytes(3 * p.n * cnt), np.uint8))
-        keys += [(pid, bytes(sk[i]), bytes(pk[i])) for i in range(cnt)]
-    pool = np.random.default_rng(seed).integers(0, 256, 1 << 20, dtype=np.uint8).tobytes()
+        seeds = [s for nm, s in kdef if nm == name]
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+        keys += [(pid, bytes(sk[i]), bytes(pk[i])) for i in range(len(seeds))]
+    pool = W.MessagePool(seed)
 
     def msg():
-        n = int(round(np.exp(rng.uniform(np.log(16), np.log(65536)))))
-        o = rng.randrange(0, len(pool) - n)
-        return pool[o:o + n]
+        return pool.draw(rng)
 
     # signatures for the verify requests: 2 messages per key
     signed = []
@@ -275,14 +348,17 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_
                                     "sign_p50_ms": round(st["sign_p50_us"] / 1e3, 3),
                                     "sign_p99_ms": round(st["sign_p99_us"] / 1e3, 3),
                                     "verify_p99_ms": round(st["verify_p99_us"] / 1e3, 3)}
-        # walk up from the fixed rate; the first rate over 50 ms ends the walk
-        best = None
-        for q, p99 in sorted([(qps, out["sign_p99_ms"])] +
-                             [(float(k), v["sign_p99_ms"]) for k, v in out["sweep"].items()]):
+        # walk up from the fixed rate; the first rate over 50 ms ends the walk.
+        # The speedup below uses the throughput achieved at that rate.
+        best = best_rps = None
+        for q, p99, rps in sorted([(qps, out["sign_p99_ms"], out["throughput_rps"])] +
+                                  [(float(k), v["sign_p99_ms"], v["throughput_rps"])
+                                   for k, v in out["sweep"].items()]):
             if p99 > 50.0:
                 break
-            best = q
+            best, best_rps = q, rps
         out["max_qps_p99_le_50ms"] = best
+        out["achieved_rps_p99_le_50ms"] = best_rps
         srv.reset_stats()
         reqs = mix(sat_requests)
         el, bad2 = run(srv, kids, reqs, [0.0] * len(reqs))
@@ -302,8 +378,8 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_
         out["per_request_launch_rps"] = round(single_requests / el, 1)
     out["invalid"] = bad
     out["speedup_vs_per_request"] = round(out["saturation_rps"] / out["per_request_launch_rps"], 2)
-    if out["max_qps_p99_le_50ms"]:
-        out["speedup_at_p99_le_50ms"] = round(out["max_qps_p99_le_50ms"] /
+    if out["achieved_rps_p99_le_50ms"]:
+        out["speedup_at_p99_le_50ms"] = round(out["achieved_rps_p99_le_50ms"] /
                                               out["per_request_launch_rps"], 2)
     return out
 
@@ -311,21 +387,19 @@ def serve_phase(C, seconds, qps, sat_requests, single_requests, slots=4, window_
 def mixed_phase(C, n, reps=3, seed=3):
     """One mixed batch of n SHAKE256-128f / -192f signing requests (half
     each, distinct DRBG keys, messages log-uniform 1 B .. 4 KiB) replayed as
-    one hipGraph; returns signs/s of the whole mixed batch."""
-    import random
+    one hipGraph (workloads.mixed_shake_batch); returns signs/s of the whole
+    mixed batch."""
     import numpy as np
-    rng = random.Random(seed)
-    C.randombytes_init(bytes([seed]) * 48)
-    items = []
+    from cuspx_amd import workloads as W
+    work = W.mixed_shake_batch(LibDrbg(C), n, seed)
+    sks = [None] * len(work)
     for name in ("shake256-128f", "shake256-192f"):
-        pid = C.param_id(name)
-        p = C.params(pid)
-        cnt = n // 2
-        pk, sk = C.batch_keygen(pid, np.frombuffer(C.randombytes(3 * p.n * cnt), np.uint8))
-        for i in range(cnt):
-            mlen = int(round(np.exp(rng.uniform(0, np.log(4096)))))
-            items.append((pid, bytes(sk[i]), rng.randbytes(mlen), C.randombytes(p.n)))
-    rng.shuffle(items)
+        idx = [i for i, it in enumerate(work) if it[0] == name]
+        pk, sk = C.batch_keygen(C.param_id(name),
+                                np.frombuffer(b"".join(work[i][1] for i in idx), np.uint8))
+        for j, i in enumerate(idx):
+            sks[i] = bytes(sk[j])
+    items = [(C.param_id(name), sk, m, opt) for (name, _, m, opt), sk in zip(work, sks)]
     mb = C.MixedSignBatch(items)
     st = C.Stream()
     mb.launch(st)
@@ -354,19 +428,19 @@ def mixed_phase(C, n, reps=3, seed=3):
 
 def graph_128s_phase(C, sizes=(1, 64, 1024), reps=3, seed=4):
     """SHA2-128s signing batches of 1 / 64 / 1024 captured once as hipGraphs
-    and replayed; returns ms per batch and signs/s per size."""
-    import random
+    and replayed (workloads.sha2_128s_batches); returns ms per batch and
+    signs/s per size."""
     import numpy as np
+    from cuspx_amd import workloads as W
     pid = C.param_id("sha2-128s")
     p = C.params(pid)
-    rng = random.Random(seed)
     out = {}
-    for B in sizes:
-        pk, sk = C.batch_keygen(pid, np.frombuffer(rng.randbytes(3 * p.n * B), np.uint8))
+    for B, (seeds, msgs, opts) in W.sha2_128s_batches(sizes, seed).items():
+        pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
         D = C.DeviceBuffer
         d_sk = D.from_array(sk)
-        d_msg = D.from_array(np.frombuffer(rng.randbytes(32 * B), np.uint8))
-        d_opt = D.from_array(np.frombuffer(rng.randbytes(p.n * B), np.uint8))
+        d_msg = D.from_array(np.frombuffer(b"".join(msgs), np.uint8))
+        d_opt = D.from_array(np.frombuffer(b"".join(opts), np.uint8))
         d_moff = D.from_array(np.arange(B, dtype=np.uint64) * 32)
         d_mlen = D.from_array(np.full(B, 32, np.uint32))
         d_sig = D(B * p.sig_bytes)
@@ -430,15 +504,27 @@ def main():
 
     cpu = None
     if not args.no_cpu:  # before the GPU is touched: the baseline forks
-        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
-        print("cpu baseline: %d processes, %.0f s" % (procs, args.cpu_seconds), file=sys.stderr,
-              flush=True)
-        cs, cv, c1 = cpu_baseline(pid, args.cpu_seconds, procs)
-        cpu = {"value": round(cs, 3), "unit": "signs/s", "cores": procs, "kind": "port",
-               "verifies_per_s": round(cv, 2), "signs_per_s_1core": round(c1, 3),
-               "sample": "oracle/spx_oracle.c %s sign (32-byte messages) for %.0f s then verify "
-                         "for %.0f s in each of %d processes" % (args.param, 0.8 * args.cpu_seconds,
-                                                                 0.2 * args.cpu_seconds, procs)}
+        # one process per physical core, capped at the 16 CPUs a one-GPU
+        # share of the host allows
+        max_procs = args.cpu_procs or 16
+        print("cpu baseline: up to %d pinned processes, %.0f s" % (max_procs, args.cpu_seconds),
+              file=sys.stderr, flush=True)
+        cb = cpu_baseline(pid, args.cpu_seconds, max_procs)
+        one = cb["single_core"]
+        cpu = {"value": round(cb["signs_per_s"], 3), "unit": "signs/s",
+               "cores": cb["processes"], "kind": "port",
+               "verifies_per_s": round(cb["verifies_per_s"], 2),
+               "cpu_model": cb["cpu_model"],
+               "physical_cores_visible": cb["physical_cores_visible"],
+               "logical_cpus_visible": cb["logical_cpus_visible"],
+               "single_core_median_ms": {k: round(v, 3) for k, v in one.items()},
+               "signs_per_s_1core": round(1e3 / one["sign_ms_monotonic"], 3),
+               "sample": "oracle/spx_oracle.c %s, 32-byte messages: one pinned process, 2 warm-up "
+                         "+ 10 timed sign / verify (medians, CLOCK_MONOTONIC and "
+                         "CLOCK_PROCESS_CPUTIME_ID); then %d processes pinned one per physical "
+                         "core, signing %.0f s then verifying %.0f s each, rates summed"
+                         % (args.param, cb["processes"], 0.8 * args.cpu_seconds,
+                            0.2 * args.cpu_seconds)}
 
     info = C.device_info()
     print("device: %s" % info, file=sys.stderr, flush=True)
@@ -498,15 +584,16 @@ def main():
 
     # the dominant kernel: the fused FORS + hypertree launch, or k_ht alone
     kname = "k_sign_fused" if "k_sign_fused" in prof else "k_ht"
-    traffic = None
+    pmc = None
     try:
-        tj = json.load(open(os.path.join(ROOT, "profiles", "pmc_traffic.json")))
-        t = tj.get("%s/%s/%d" % (kname, p.name.decode(), B))
-        if t:
-            traffic = int(1024 * (t["fetch_kib"] * t["fetch_scale"] +
-                                  t["write_kib"] * t["write_scale"]))
+        tj = json.load(open(os.path.join(ROOT, "profiles", "pmc_counters.json")))
+        pmc = tj.get("%s/%s/%d" % (kname, p.name.decode(), B))
     except (OSError, ValueError):
         pass
+    traffic = None
+    if pmc:
+        traffic = int(1024 * (pmc["fetch_kib"] * pmc["fetch_scale"] +
+                              pmc["write_kib"] * pmc["write_scale"]))
     host = None
     if args.host_reps > 0:
         print("host-buffer entry points: %d reps" % args.host_reps, file=sys.stderr, flush=True)
@@ -530,7 +617,14 @@ def main():
     kt, kn = prof.get(kname, (0.0, 0))
     k_avg = kt / max(kn, 1) / 1e3
     work = kernel_work(p, B) + (fors_work(p, B) if kname == "k_sign_fused" else 0)
-    ops_unit = SHA256_OPS if p.hash == 0 else KECCAK_OPS
+    # lane-instructions per unit: the launch's SQ_INSTS_VALU x 64 over its
+    # units, as counted for this kernel and batch (profiles/pmc_counters.json)
+    if pmc and pmc.get("sq_insts_valu"):
+        ops_unit = round(pmc["sq_insts_valu"] * 64 / work)
+        ops_source = "SQ_INSTS_VALU x 64 / units, profiles/pmc_counters.json"
+    else:
+        ops_unit = SHA256_OPS if p.hash == 0 else KECCAK_OPS
+        ops_source = "constant"
     units_s = work / k_avg if k_avg > 0 else 0.0
     achieved = units_s * ops_unit / 1e12
     ceiling = mix_ceiling_units(MIX_RATE_SHA256 if p.hash == 0 else MIX_RATE_KECCAK, ops_unit)
@@ -569,6 +663,7 @@ def main():
                      "work_per_launch": work,
                      "work_unit": "sha256 compressions" if p.hash == 0 else "keccak-f permutations",
                      "ops_per_unit": ops_unit,
+                     "ops_per_unit_source": ops_source,
                      "kernel_ms": round(k_avg * 1e3, 3),
                      "units_per_s": round(units_s, 1),
                      "frac_at_1450_ops": round(units_s / (PEAK_VOP3_T * 1e12 / SHA256_OPS_SURVEY), 4)

@@ -4,8 +4,7 @@
     python -m cuspx_amd.build --force    # rebuild everything
 
 Each parameter set is its own translation unit (csrc/set_*.hip) so the
-twelve instantiations compile in parallel. It also builds the measurement
-probes (probe/valu_rate.hip, probe/chain_latency.hip -> cuspx_amd/<name>).
+twelve instantiations compile in parallel.
 """
 import concurrent.futures as cf
 import glob
@@ -17,8 +16,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuspx.so")
-# measurement probes: probe/<name>.hip -> cuspx_amd/<name>
-PROBES = ("valu_rate", "chain_latency")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -32,7 +29,7 @@ FLAGS = [
 
 
 def _deps_mtime():
-    hs = glob.glob(os.path.join(CSRC, "*.hpp")) + [os.path.join(ROOT, "include", "cuspx.h")]
+    hs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
     return max(os.path.getmtime(h) for h in hs)
 
 
@@ -70,14 +67,6 @@ def build(force=False, jobs=None, verbose=False):
         if p.returncode != 0:
             raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
         os.replace(tmp, LIB)
-    for name in PROBES:
-        src, exe = os.path.join(HERE, "probe", name + ".hip"), os.path.join(HERE, name)
-        if force or not os.path.exists(exe) or os.path.getmtime(exe) < max(
-                os.path.getmtime(src), _deps_mtime()):
-            cmd = [HIPCC] + FLAGS + ["-I" + CSRC, "-o", exe, src]
-            p = subprocess.run(cmd, capture_output=True, text=True)
-            if p.returncode != 0:
-                raise RuntimeError("hipcc failed for %s.hip:\n%s" % (name, p.stdout + p.stderr))
     return LIB
 
 
