@@ -632,10 +632,25 @@ def graph_sign_mixed(groups):
     return Graph(h.value)
 
 
+def sign_mixed_device(groups, stream=None):
+    """Launch a mixed batch (list of SignGroup) directly on `stream`: one
+    persistent kernel when the groups are distinct fast sets of one hash
+    family, else the groups one after another."""
+    L = lib()
+    L.cuspx_sign_mixed_device.argtypes = [ctypes.POINTER(SignGroup), ctypes.c_int,
+                                          ctypes.c_void_p]
+    arr = (SignGroup * len(groups))(*groups)
+    return _check(L.cuspx_sign_mixed_device(arr, len(groups), stream.handle if stream else None),
+                  "cuspx_sign_mixed_device")
+
+
 class MixedSignBatch:
     """A resident mixed-parameter-set signing batch: requests are grouped by
     set, each group gets device buffers and a workspace, and the whole batch
-    is one hipGraph (one launch per replay, the sets run concurrently).
+    is one hipGraph (one launch per replay).  Distinct fast sets of one hash
+    family run as one persistent kernel; other mixes run each set's pipeline
+    on its own forked stream.  launch(eager=True) launches the same batch
+    without the graph.
 
     items: list of (pid, sk bytes, message bytes, optrand bytes)."""
 
@@ -671,10 +686,14 @@ class MixedSignBatch:
             descs.append(SignGroup(pid, num, d["sk"].ptr, p.sk_bytes, d["msg"].ptr, d["moff"].ptr,
                                    d["mlen"].ptr, d["opt"].ptr, d["sig"].ptr, p.sig_bytes,
                                    d["ws"].ptr))
+        self.descs = descs
         self.graph = graph_sign_mixed(descs)
 
-    def launch(self, stream=None):
-        self.graph.launch(stream)
+    def launch(self, stream=None, eager=False):
+        if eager:
+            sign_mixed_device(self.descs, stream)
+        else:
+            self.graph.launch(stream)
 
     def signatures(self):
         out = [None] * self.n

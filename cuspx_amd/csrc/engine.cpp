@@ -37,6 +37,8 @@ const spx::SetOps* cuspx_set_shake256_192s(void);
 const spx::SetOps* cuspx_set_shake256_192f(void);
 const spx::SetOps* cuspx_set_shake256_256s(void);
 const spx::SetOps* cuspx_set_shake256_256f(void);
+const spx::FamilyOps* cuspx_family_sha2_f(void);
+const spx::FamilyOps* cuspx_family_shake256_f(void);
 }
 
 namespace spx {
@@ -1066,36 +1068,94 @@ int cuspx_graph_verify(int pid, uint32_t num, const uint8_t* pk, uint32_t pk_str
     return capture([&](hipStream_t s) { return o->verify(b, s); }, out);
 }
 
+}  // extern "C"
+
+static DevBatch group_batch(const cuspx_sign_group& gr) {
+    DevBatch b{};
+    b.n = gr.num;
+    b.sk = gr.sk;
+    b.sk_stride = gr.sk_stride;
+    b.msg = gr.msg;
+    b.moff = gr.moff;
+    b.mlen = gr.mlen;
+    b.optrand = gr.optrand;
+    b.sig = gr.sig;
+    b.sig_stride = gr.sig_stride;
+    b.ws = gr.ws;
+    return b;
+}
+
+static int check_groups(const cuspx_sign_group* groups, int ngroups) {
+    if (!groups || ngroups <= 0 || ngroups > 64) return fail(kErrArg, "bad group list%s");
+    for (int i = 0; i < ngroups; i++) {
+        const cuspx_sign_group& g = groups[i];
+        if (!get_set(g.pid)) return kErrParam;
+        if (g.num && (!g.sk || !g.msg || !g.moff || !g.mlen || !g.optrand || !g.sig || !g.ws))
+            return fail(kErrArg, "null argument in a group%s");
+    }
+    return kOk;
+}
+
+// A mixed batch whose non-empty groups are distinct fast sets of one hash
+// family runs as one persistent kernel (FamilyOps); returns kNotFused if the
+// groups do not form such a batch or a group would not run at one lane per
+// WOTS leaf (nothing launched then).
+static int sign_family(const cuspx_sign_group* groups, int ngroups, hipStream_t s) {
+    const FamilyOps* fams[2] = {cuspx_family_sha2_f(), cuspx_family_shake256_f()};
+    for (const FamilyOps* f : fams) {
+        DevBatch b[3] = {};
+        bool ok = true, any = false;
+        for (int i = 0; i < ngroups && ok; i++) {
+            if (!groups[i].num) continue;
+            int slot = -1;
+            for (int k = 0; k < 3; k++)
+                if (f->pid[k] == groups[i].pid) slot = k;
+            ok = slot >= 0 && b[slot].n == 0;
+            if (ok) b[slot] = group_batch(groups[i]);
+            any = true;
+        }
+        if (ok && any) return f->sign(b, s);
+    }
+    return kNotFused;
+}
+
+extern "C" {
+
+int cuspx_sign_mixed_device(const cuspx_sign_group* groups, int ngroups, void* stream) {
+    int rc = check_groups(groups, ngroups);
+    if (rc) return rc;
+    hipStream_t s = pick_stream(stream);
+    rc = sign_family(groups, ngroups, s);
+    if (rc != kNotFused) return rc;
+    for (int i = 0; i < ngroups; i++) {  // one set after another on the stream
+        if (!groups[i].num) continue;
+        rc = get_set(groups[i].pid)->sign(group_batch(groups[i]), s);
+        if (rc) return rc;
+    }
+    return kOk;
+}
+
 int cuspx_graph_sign_mixed(const cuspx_sign_group* groups, int ngroups, cuspx_graph** out) {
-    if (!groups || ngroups <= 0 || ngroups > 64) return kErrArg;
-    for (int i = 0; i < ngroups; i++)
-        if (!get_set(groups[i].pid)) return kErrParam;
+    int rc0 = check_groups(groups, ngroups);
+    if (rc0) return rc0;
     std::vector<hipStream_t> fs(ngroups, nullptr);
     std::vector<hipEvent_t> ev(ngroups + 1, nullptr);
     for (int i = 0; i < ngroups; i++) HIP_OK(hipStreamCreateWithFlags(&fs[i], hipStreamNonBlocking));
     for (auto& e : ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // fork: every group waits on the capture stream, runs on its own stream,
-    // and the capture stream joins all of them
     int rc = capture(
         [&](hipStream_t s) {
+            const int r = sign_family(groups, ngroups, s);
+            if (r != kNotFused) return r;
+            // other mixes: every group waits on the capture stream, runs its
+            // own pipeline on its own stream, and the capture stream joins
+            // all of them
             if (hipEventRecord(ev[ngroups], s) != hipSuccess) return (int)kErrHip;
             for (int i = 0; i < ngroups; i++) {
                 const cuspx_sign_group& gr = groups[i];
                 if (hipStreamWaitEvent(fs[i], ev[ngroups], 0) != hipSuccess) return (int)kErrHip;
-                DevBatch b{};
-                b.n = gr.num;
-                b.sk = gr.sk;
-                b.sk_stride = gr.sk_stride;
-                b.msg = gr.msg;
-                b.moff = gr.moff;
-                b.mlen = gr.mlen;
-                b.optrand = gr.optrand;
-                b.sig = gr.sig;
-                b.sig_stride = gr.sig_stride;
-                b.ws = gr.ws;
                 if (gr.num) {
-                    const int r = get_set(gr.pid)->sign(b, fs[i]);
-                    if (r) return r;
+                    const int r2 = get_set(gr.pid)->sign(group_batch(gr), fs[i]);
+                    if (r2) return r2;
                 }
                 if (hipEventRecord(ev[i], fs[i]) != hipSuccess) return (int)kErrHip;
                 if (hipStreamWaitEvent(s, ev[i], 0) != hipSuccess) return (int)kErrHip;

@@ -61,6 +61,18 @@ struct SetOps {
 
 const SetOps* get_set(int pid);
 
+// returned by FamilyOps::sign when a batch does not fit the fused path
+// (nothing was launched)
+constexpr int kNotFused = 1;
+
+// The fast sets of one hash family (128f, 192f, 256f) signed as one mixed
+// batch by one persistent kernel (family_impl.hpp).
+struct FamilyOps {
+    int pid[3];
+    // Signs b[0..2] (one batch per set, in the order of pid; n may be 0).
+    int (*sign)(const DevBatch* b, hipStream_t s);
+};
+
 // sets the text cuspx_last_error() reports on this thread; returns code
 int set_error(int code, const char* msg);
 

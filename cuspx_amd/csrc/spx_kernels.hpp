@@ -261,11 +261,19 @@ __device__ __forceinline__ void fors_root_done(const BatchArgs& A, uint32_t req)
     }
 }
 
+// LDS words a FORS tile of set P uses (two levels of 256 nodes)
+template <class P>
+constexpr int fors_lds_words() {
+    return 2 * 256 * P::nw;
+}
+
 // sign: FORS trees, L = 2^(a-J) lanes per tree, 2^J leaves per lane; `blk`
 // is the workgroup's tile (TPW trees), so the same body serves k_fors and the
-// fused persistent signing kernel
+// fused persistent signing kernels.  smem: fors_lds_words<P>() words of LDS
+// owned by the kernel (shared with the hypertree tiles of a persistent
+// kernel, which never run at the same time in one workgroup)
 template <class P, int J>
-__device__ __forceinline__ void fors_tile(const BatchArgs& A, uint32_t blk) {
+__device__ __forceinline__ void fors_tile(const BatchArgs& A, uint32_t blk, uint32_t* smem) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     constexpr int NW = P::nw;
@@ -274,7 +282,7 @@ __device__ __forceinline__ void fors_tile(const BatchArgs& A, uint32_t blk) {
     constexpr int WG = 256;
     static_assert(L <= WG, "tree wider than a workgroup");
     constexpr int TPW = WG / L;
-    __shared__ uint32_t lds[2][WG][NW];
+    auto& lds = *reinterpret_cast<uint32_t (*)[2][WG][NW]>(smem);
 
     const int tid = threadIdx.x;
     const uint32_t g = blk * TPW + tid / L;  // global tree id
@@ -374,7 +382,8 @@ __device__ __forceinline__ void fors_tile(const BatchArgs& A, uint32_t blk) {
 
 template <class P, int J>
 __global__ void __launch_bounds__(256) k_fors(BatchArgs A) {
-    fors_tile<P, J>(A, blockIdx.x);
+    __shared__ uint32_t smem[fors_lds_words<P>()];
+    fors_tile<P, J>(A, blockIdx.x, smem);
 }
 
 // thash (SHA-256) over M payload words after the 22-byte address,
@@ -877,10 +886,13 @@ struct HtShape {
     static constexpr int TPW = PARTS > 1 ? 1 : WG / LT;  // subtrees per workgroup
     static constexpr int NL = WG / S;                 // leaves per workgroup
     static constexpr bool park = S > 1 || P::hash == kShake;
+    static constexpr int NLDS = NL > PARTS ? NL : PARTS;  // the last part climbs from PARTS roots
+    static constexpr int lds_words = 2 * NLDS * P::nw;    // two levels of nodes
 };
 
+// smem: HtShape<P, S>::lds_words words of LDS owned by the kernel
 template <class P, int MODE, int S>
-__device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
+__device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk, uint32_t* smem) {
     using Hh = HashT<P>;
     using Key = typename Hh::Key;
     using Sh = HtShape<P, S>;
@@ -889,8 +901,8 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
     constexpr int L = Sh::L, TPW = Sh::TPW, NL = Sh::NL;
     constexpr int PARTS = Sh::PARTS, LP = Sh::LP, LOGP = Sh::LOGP;
     constexpr int LPT = PARTS > 1 ? LP * S : Sh::LT;  // lanes per subtree in this workgroup
-    constexpr int NLDS = NL > PARTS ? NL : PARTS;  // the last part climbs from PARTS roots
-    __shared__ uint32_t lds[2][NLDS][NW];
+    constexpr int NLDS = Sh::NLDS;
+    auto& lds = *reinterpret_cast<uint32_t (*)[2][NLDS][NW]>(smem);
 
     const int tid = threadIdx.x;
     const int tau = tid / LPT;
@@ -1054,8 +1066,11 @@ __device__ __forceinline__ void ht_tile(const BatchArgs& A, uint32_t blk) {
 
 template <class P, int MODE, int S>
 __global__ void __launch_bounds__((HtShape<P, S>::WG)) k_ht(BatchArgs A) {
-    ht_tile<P, MODE, S>(A, blockIdx.x);
+    __shared__ uint32_t smem[HtShape<P, S>::lds_words];
+    ht_tile<P, MODE, S>(A, blockIdx.x, smem);
 }
+
+constexpr int max_int(int a, int b) { return a > b ? a : b; }
 
 // Signing as one persistent kernel: a grid of resident 256-lane workgroups
 // pulls tiles from a device counter (zeroed by k_prep) until the queue is
@@ -1070,6 +1085,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::has
 k_sign_fused(BatchArgs A, uint32_t n_ht, uint32_t n_fors) {
     static_assert(HtShape<P, S>::WG == 256, "hypertree tiles of 256 lanes");
     __shared__ uint32_t tile;
+    // one LDS buffer for both tile kinds (a workgroup runs one tile at a time)
+    __shared__ uint32_t smem[max_int(HtShape<P, S>::lds_words, fors_lds_words<P>())];
 #pragma unroll 1
     while (true) {
         if (threadIdx.x == 0) tile = atomicAdd(A.tile_ctr, 1u);
@@ -1077,8 +1094,8 @@ k_sign_fused(BatchArgs A, uint32_t n_ht, uint32_t n_fors) {
         const uint32_t t = tile;
         __syncthreads();  // every lane has the tile (and left the previous one)
         if (t >= n_ht + n_fors) break;
-        if (t < n_ht) ht_tile<P, 0, S>(A, t);
-        else fors_tile<P, J>(A, t - n_ht);
+        if (t < n_ht) ht_tile<P, 0, S>(A, t, smem);
+        else fors_tile<P, J>(A, t - n_ht, smem);
     }
 }
 
@@ -1252,7 +1269,11 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
     __shared__ uint32_t pk_s[R][TLW];   // chain ends (SHAKE256: interleaved)
     __shared__ uint32_t auth_s[R][AW];  // auth path of the layer, raw words
     __shared__ uint32_t node_s[R][NW];  // climb: the node, raw words
-    __shared__ uint32_t kw_s[PRE ? R * MB : 1][64];  // K + W of T_len's payload-only blocks
+    // K + W of T_len's payload-only blocks: 64-word rows, row (r, b) skewed
+    // by 4 words per request so the pairs' 16-byte reads of one t (one row
+    // per request) fall into distinct bank groups
+    __shared__ __align__(16) uint32_t kw_s[PRE ? R * MB * 64 + 64 : 1];
+    auto kw_row = [&](int r, int b) { return &kw_s[(r * MB + b) * 64 + 4 * (r & 15)]; };
     __shared__ uint16_t order_s[NCH];   // rank -> chain id | remaining steps << 12
     __shared__ uint32_t hist_s[16];     // chains per remaining-step count
 
@@ -1430,14 +1451,14 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
                 SP::init(k2.mid, st);
                 SP::absorb(st, 0, 1, pa, rpk);
             }
-            if (tid >= 64) {
+            if (tid >= 64) {  // consecutive lanes take consecutive requests
                 for (int it = tid - 64; it < nact * MB; it += 192)
-                    sha2_kw_block(&pk_s[it / MB][0], 1 + it % MB, &kw_s[it][0]);
+                    sha2_kw_block(&pk_s[it % nact][0], 1 + it / nact, kw_row(it % nact, it / nact));
             }
             __syncthreads();  // K + W of every payload-only block in LDS
             if (tid < 2 * nact) {
 #pragma unroll 1
-                for (int b = 0; b < MB; b++) sha256_rounds_pair_kw(st, &kw_s[rq * MB + b][0]);
+                for (int b = 0; b < MB; b++) sha256_rounds_pair_kw(st, kw_row(rq, b));
                 SP::absorb(st, SP::FT, SP::B, pa, rpk);
                 uint32_t node[NW];
                 SP::output(st, node);

@@ -169,8 +169,14 @@ int cuspx_graph_verify(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_str
                        const uint8_t *sig, uint64_t sig_stride, const uint8_t *msg,
                        const uint64_t *moff, const uint32_t *mlen, int32_t *status, void *ws,
                        cuspx_graph **out);
-/* a mixed batch: `groups` single-set sign batches, each on its own forked
- * stream inside one graph (one launch signs every set concurrently) */
+/* A mixed batch: `groups` single-set sign batches (device-resident, as in
+ * cuspx_sign_device; each with its own workspace).  When the non-empty
+ * groups are distinct fast sets of one hash family (sha2-128f/192f/256f or
+ * shake256-128f/192f/256f) and each runs at one lane per WOTS leaf, they
+ * are signed by ONE persistent kernel whose tile queue spans every set
+ * (after each set's prep kernel; T_k and the WOTS gather per set after it).
+ * Other mixes run each group's own pipeline on a forked stream.  Not in the
+ * reference (its batches are single-set, sign.cu:5098). */
 typedef struct {
     int pid;
     uint32_t num;
@@ -185,6 +191,9 @@ typedef struct {
     void *ws;
 } cuspx_sign_group;
 int cuspx_graph_sign_mixed(const cuspx_sign_group *groups, int ngroups, cuspx_graph **out);
+/* the same mixed batch launched directly on `stream` (the persistent kernel
+ * when it applies, else the groups one after another) */
+int cuspx_sign_mixed_device(const cuspx_sign_group *groups, int ngroups, void *stream);
 int cuspx_graph_launch(cuspx_graph *g, void *stream);
 int cuspx_graph_destroy(cuspx_graph *g);
 

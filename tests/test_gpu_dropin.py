@@ -29,7 +29,8 @@ def _run(name, *args):
     assert os.path.exists(exe), "build the drop-in callers first (make -C tests/dropin)"
     print("running %s %s" % (name, " ".join(args)), flush=True)
     p = subprocess.run([exe] + list(args), capture_output=True, text=True, timeout=300)
-    assert p.returncode == 0 and "\nOK " in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    ok = p.stdout.startswith("OK ") or "\nOK " in p.stdout
+    assert p.returncode == 0 and ok, p.stdout[-2000:] + p.stderr[-2000:]
     return p.stdout
 
 

@@ -129,11 +129,12 @@ def test_server_refusals():
         assert e.value.code == C.ERR_ARG
         assert srv.wait(srv.submit_verify(vk, b"verify-only", bytes(sig[0])))[0] == 0
         assert srv.wait(srv.submit_verify(vk, b"verify-Only", bytes(sig[0])))[0] == -1
-    # 8 MiB over 4 slots: a SHA2-128s signing workspace (about 2 MiB) never fits
+    # 4 MiB over 4 slots: a SHA2-128s signing workspace (about 2 MiB) never fits
     pid_s = C.param_id("sha2-128s")
     ps = O.params(pid_s)
+    assert C.workspace_bytes(pid_s, 1, C.OP_SIGN) > (1 << 20)
     pk_s, sk_s = C.batch_keygen(pid_s, np.frombuffer(bytes(3 * ps.n), np.uint8))
-    with C.Server(hbm_budget_bytes=8 << 20, slots=4) as srv:
+    with C.Server(hbm_budget_bytes=4 << 20, slots=4) as srv:
         k = srv.add_key(pid_s, bytes(sk_s[0]), bytes(pk_s[0]))
         with pytest.raises(C.CuspxError) as e:
             srv.submit_sign(k, b"m")
