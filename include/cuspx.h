@@ -109,10 +109,10 @@ int face_mdp_crypto_sign_open(unsigned char *m, unsigned long long *mlen, const 
  * sign.cu:1400, api.h:112 face_ms_mdp_crypto_sign / sign.cu:5305 and
  * api.h:151 face_ms_mdp_crypto_sign_open / sign.cu:7080 (the reference's
  * non-const pointers).  Same arguments, outputs and DRBG draws as the
- * face_mdp_* calls above; the batch is cut into up to 4 slices on separate
- * streams whose kernels run in order while one slice's PCIe copies and host
- * staging overlap the next slice's kernels (slices of >= 2048 items for
- * keygen / sign, >= 1024 for verify; smaller batches run as one slice). */
+ * face_mdp_* calls above; the batch is cut into up to 4 slices that run
+ * concurrently on separate streams, so one slice's PCIe copies and host
+ * staging overlap the other slices' kernels (slices of >= 2048 items;
+ * smaller batches run as one slice). */
 int face_ms_mdp_crypto_sign_keypair(unsigned char *pk, unsigned char *sk, unsigned int num);
 int face_ms_mdp_crypto_sign(unsigned char *sm, unsigned long long *smlen, unsigned char *m,
                             unsigned long long mlen, unsigned char *sk, unsigned int num);
