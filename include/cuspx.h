@@ -154,6 +154,20 @@ int cuspx_verify_device(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_st
                         void *stream);
 int cuspx_keygen_device(int pid, uint32_t num, const uint8_t *seed, uint32_t seed_stride,
                         uint8_t *pk, uint8_t *sk, void *ws, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cuspx_sign_device without the workspace argument.  The
+ * workspace is a 128 MiB array the library holds on each GPU until the
+ * process ends; the call allocates nothing.  A batch whose workspace
+ * (cuspx_workspace_bytes(pid, num, 0)) exceeds 128 MiB is refused with -5
+ * before any launch (up to 451 sha2-128f, 114 sha2-256f, 64 sha2-128s
+ * requests).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race and must be
+ * serialised by the caller.  stream NULL = HIP's null stream. */
+int cuspx_sign_device_noalloc(int pid, uint32_t num, const uint8_t *sk, uint32_t sk_stride,
+                              const uint8_t *msg, const uint64_t *moff, const uint32_t *mlen,
+                              const uint8_t *optrand, uint8_t *sig, uint64_t sig_stride,
+                              void *stream);
 
 /* ------------------------------------------------------------ hipGraph replay
  * Capture the launch sequence of one device-resident batch (the arguments of
