@@ -423,7 +423,9 @@ def mixed_phase(C, n, reps=3, seed=3):
     return {"sets": ["shake256-128f", "shake256-192f"], "requests": len(items),
             "message_bytes": "log-uniform 1..4096", "ms_per_batch": round(ms, 3),
             "signs_per_s": round(len(items) / ms * 1e3, 1), "spot_check_invalid": bad,
-            "launch": "one hipGraph per replay (per-set pipelines on forked streams)"}
+            "launch": "one hipGraph per replay: both sets' prep kernels, one persistent "
+                      "family kernel over every set's tiles, then T_k and the WOTS gather "
+                      "per set"}
 
 
 def graph_128s_phase(C, sizes=(1, 64, 1024), reps=3, seed=4):
