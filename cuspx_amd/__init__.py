@@ -501,6 +501,13 @@ class Server:
         self._sig_bytes[kid.value] = params(pid).sig_bytes
         return kid.value
 
+    def remove_key(self, key):
+        """Refuse new requests on the key; queued / running ones finish and
+        its row is reused afterwards (under a new key id)."""
+        L = lib()
+        L.spx_server_remove_key.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        return _check(L.spx_server_remove_key(self.h, key), "spx_server_remove_key")
+
     def _submit(self, op, key, msg, aux):
         t = ctypes.c_uint64()
         m = np.frombuffer(msg, np.uint8) if len(msg) else np.zeros(1, np.uint8)

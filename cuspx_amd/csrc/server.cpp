@@ -63,12 +63,36 @@ struct Request {
 };
 using ReqP = std::shared_ptr<Request>;
 
+// Key ids are (generation << 20) | (set << 16) | row: a removed key's row is
+// reused with the next generation, so a stale id cannot reach the new key.
+constexpr int32_t kRowBits = 16, kSetBits = 4, kGenMask = (1 << 11) - 1;
+inline int32_t make_key_id(int pid, uint32_t row, uint32_t gen) {
+    return (int32_t)(((gen & kGenMask) << (kRowBits + kSetBits)) | ((uint32_t)pid << kRowBits) |
+                     row);
+}
+
 struct KeyTable {
     uint8_t* d_sk = nullptr;
     uint8_t* d_pk = nullptr;
     uint32_t* d_mid = nullptr;
-    std::vector<bool> has_sk;  // rows added without a secret key verify only
-    uint32_t count = 0, cap = 0;
+    enum : uint8_t { kFree = 0, kLive = 1, kRemoved = 2 };
+    std::vector<uint8_t> state;    // per row
+    std::vector<bool> has_sk;      // rows added without a secret key verify only
+    std::vector<uint32_t> refs;    // requests queued or running on the row
+    std::vector<uint32_t> gen;     // generation of the row's current key
+    std::vector<uint32_t> free_rows;  // removed rows whose requests have drained
+    uint32_t count = 0, cap = 0;   // rows ever used, capacity
+
+    bool live(uint32_t row, uint32_t g) const {
+        return row < count && state[row] == kLive && (gen[row] & kGenMask) == g;
+    }
+    // a request on `row` finished (or left its queue)
+    void release(uint32_t row) {
+        if (row < count && refs[row] && --refs[row] == 0 && state[row] == kRemoved) {
+            state[row] = kFree;
+            free_rows.push_back(row);
+        }
+    }
 };
 
 struct Slot {
@@ -194,8 +218,10 @@ int Server::launch(Slot& sl, int pid, int op, std::vector<ReqP>& reqs) {
 void Server::harvest(Slot& sl) {
     const SetOps* o = get_set(sl.pid);
     const auto now = Clock::now();
+    KeyTable& kt = keys[sl.pid];
     for (size_t i = 0; i < sl.batch.size(); i++) {
         Request& r = *sl.batch[i];
+        kt.release(r.row);
         if (sl.op == 0) {
             r.sig.assign(sl.h + sl.out_off + (uint64_t)o->sig_bytes * i,
                          sl.h + sl.out_off + (uint64_t)o->sig_bytes * (i + 1));
@@ -297,6 +323,7 @@ void Server::loop() {
                     r->status = rc;
                     r->done = true;
                     r->t_done = Clock::now();
+                    keys[b_pid].release(r->row);
                 }
                 cv_done.notify_all();
                 err = rc;
@@ -329,6 +356,10 @@ int spx_server_create(const spx_server_config* cfg, spx_server** out) {
     if (!s->cfg.long_slots || s->cfg.long_slots > s->cfg.slots) s->cfg.long_slots = s->cfg.slots;
     if (!s->cfg.max_msg_bytes) s->cfg.max_msg_bytes = 65536;
     if (!s->cfg.max_keys_per_set) s->cfg.max_keys_per_set = 1024;
+    if (s->cfg.max_keys_per_set > (1u << kRowBits)) {  // rows are 16 bits of a key id
+        delete s;
+        return kErrArg;
+    }
     if (!s->cfg.hbm_budget_bytes) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -377,9 +408,16 @@ int spx_server_add_key(spx_server* sv, int pid, const uint8_t* sk, const uint8_t
             hipMalloc((void**)&kt.d_mid, (uint64_t)kt.cap * 32) != hipSuccess)
             return kErrHip;
     }
-    if (kt.count >= kt.cap) return kErrBudget;
-    const uint32_t row = kt.count;
-    if (kt.has_sk.size() < kt.cap) kt.has_sk.resize(kt.cap, false);
+    if (kt.state.size() < kt.cap) {
+        kt.state.resize(kt.cap, KeyTable::kFree);
+        kt.has_sk.resize(kt.cap, false);
+        kt.refs.resize(kt.cap, 0);
+        kt.gen.resize(kt.cap, 0);
+    }
+    // a removed row whose requests have drained, else a new one
+    const bool reuse = !kt.free_rows.empty();
+    if (!reuse && kt.count >= kt.cap) return kErrBudget;
+    const uint32_t row = reuse ? kt.free_rows.back() : kt.count;
     std::vector<uint8_t> skb(o->sk_bytes, 0);
     if (sk) memcpy(skb.data(), sk, o->sk_bytes);
     if (hipMemcpy(kt.d_sk + (uint64_t)row * o->sk_bytes, skb.data(), o->sk_bytes,
@@ -390,9 +428,34 @@ int spx_server_add_key(spx_server* sv, int pid, const uint8_t* sk, const uint8_t
     int rc = o->midstates(kt.d_pk + (uint64_t)row * o->pk_bytes, o->pk_bytes, kt.d_mid + row * 8,
                           1, s->setup);
     if (rc || hipStreamSynchronize(s->setup) != hipSuccess) return kErrHip;
+    if (reuse) {
+        kt.free_rows.pop_back();
+        kt.gen[row]++;
+    } else {
+        kt.count++;
+    }
     kt.has_sk[row] = sk != nullptr;
-    kt.count++;
-    *key_id = pid * 65536 + (int32_t)row;
+    kt.state[row] = KeyTable::kLive;
+    kt.refs[row] = 0;
+    *key_id = make_key_id(pid, row, kt.gen[row]);
+    return kOk;
+}
+
+int spx_server_remove_key(spx_server* sv, int32_t key_id) {
+    Server* s = (Server*)sv;
+    if (!s || key_id < 0) return kErrArg;
+    const int pid = (key_id >> kRowBits) & ((1 << kSetBits) - 1);
+    const uint32_t row = (uint32_t)key_id & ((1u << kRowBits) - 1);
+    const uint32_t g = (uint32_t)key_id >> (kRowBits + kSetBits);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->keys.find(pid);
+    if (it == s->keys.end() || !it->second.live(row, g)) return kErrArg;
+    KeyTable& kt = it->second;
+    kt.state[row] = KeyTable::kRemoved;  // no new requests; queued / running ones finish
+    if (kt.refs[row] == 0) {
+        kt.state[row] = KeyTable::kFree;
+        kt.free_rows.push_back(row);
+    }
     return kOk;
 }
 
@@ -400,10 +463,12 @@ int spx_server_submit(spx_server* sv, int op, int32_t key_id, const uint8_t* msg
                       const uint8_t* aux, uint64_t* ticket) {
     Server* s = (Server*)sv;
     if (!s || !ticket || (op != 0 && op != 1) || (mlen && !msg)) return kErrArg;
-    const int pid = key_id / 65536;
-    const uint32_t row = (uint32_t)(key_id % 65536);
+    if (key_id < 0) return kErrParam;
+    const int pid = (key_id >> kRowBits) & ((1 << kSetBits) - 1);
+    const uint32_t row = (uint32_t)key_id & ((1u << kRowBits) - 1);
+    const uint32_t g = (uint32_t)key_id >> (kRowBits + kSetBits);
     const SetOps* o = get_set(pid);
-    if (!o || key_id < 0) return kErrParam;
+    if (!o) return kErrParam;
     if (op == 1 && !aux) return kErrArg;
     if (mlen > s->cfg.max_msg_bytes) return kErrBudget;
     auto r = std::make_shared<Request>();
@@ -425,7 +490,7 @@ int spx_server_submit(spx_server* sv, int op, int32_t key_id, const uint8_t* msg
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->keys.find(pid);
-        if (it == s->keys.end() || row >= it->second.count) return kErrArg;
+        if (it == s->keys.end() || !it->second.live(row, g)) return kErrArg;  // unknown / removed
         if (op == 0 && !it->second.has_sk[row]) return kErrArg;  // verify-only key
         // one request must fit a slot; the queue holds at most one budget of bytes
         const uint64_t need = s->item_bytes(o, op) + al16(mlen) + 8192;
@@ -440,6 +505,7 @@ int spx_server_submit(spx_server* sv, int op, int32_t key_id, const uint8_t* msg
         r->cls = cls;
         s->queues[std::make_tuple(pid, op, cls)].push_back(r);
         s->live[r->ticket] = r;
+        it->second.refs[row]++;  // the row stays until this request is done
         *ticket = r->ticket;
     }
     s->cv_work.notify_one();
@@ -485,6 +551,7 @@ int spx_server_cancel(spx_server* sv, uint64_t ticket) {
                 }
         }
         s->pending_bytes -= r->msg.size() + r->aux.size();
+        s->keys[r->pid].release(r->row);
         r->status = kErrArg;
         r->done = true;
         r->t_done = Clock::now();

@@ -225,7 +225,7 @@ typedef struct {
     uint32_t slots;             /* batches in flight (0: 4) */
     uint32_t max_msg_bytes;     /* largest message accepted (0: 65536) */
     uint32_t batch_window_us;   /* longest wait for a partial batch while the GPU is busy */
-    uint32_t max_keys_per_set;  /* resident keys per parameter set (0: 1024) */
+    uint32_t max_keys_per_set;  /* resident keys per parameter set (0: 1024, at most 65536) */
     uint32_t long_slots;        /* most slots running batches of messages over 16 KiB
                                    at once (0: all slots); fewer trades p99 and peak
                                    rate for p50, since short requests then never
@@ -243,10 +243,15 @@ typedef struct {
 typedef struct spx_server spx_server;
 int spx_server_create(const spx_server_config *cfg, spx_server **out);
 /* sk may be NULL for a verify-only key (sign requests on it are refused with
- * -4); key_id = pid * 65536 + row.  At most max_keys_per_set keys per set
- * (-5 beyond); keys stay resident until spx_server_destroy. */
+ * -4).  key_id is opaque (set, row and a generation).  At most
+ * max_keys_per_set keys per set are resident at once (-5 beyond). */
 int spx_server_add_key(spx_server *s, int pid, const uint8_t *sk, const uint8_t *pk,
                        int32_t *key_id);
+/* New requests on the key are refused (-4) from now on; requests already
+ * queued or running on it complete normally, and its row is reused by a
+ * later spx_server_add_key once they have (under a new key_id).  -4 for an
+ * unknown or already removed key. */
+int spx_server_remove_key(spx_server *s, int32_t key_id);
 /* op 0 = sign (aux = optrand[n] or NULL to draw it like crypto_sign_signature),
  * op 1 = verify (aux = signature).  -5 if the request can never fit one
  * slot's share of the budget, or the queue already holds a budget of bytes */

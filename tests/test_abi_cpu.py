@@ -148,3 +148,11 @@ def test_autotune_table_compaction():
             ("s", "verify", 4096, 16)]
     assert compact(rows) == [("s", "sign", 0, 8), ("s", "sign", 352, 4), ("s", "sign", 5632, 1),
                              ("s", "verify", 0, 64), ("s", "verify", 4096, 16)]
+
+
+def test_server_refuses_more_rows_than_a_key_id_holds():
+    """Key ids keep the row in 16 bits: a server asked for more resident keys
+    per set is refused before it touches the GPU."""
+    with pytest.raises(C.CuspxError) as e:
+        C.Server(hbm_budget_bytes=1 << 30, max_keys_per_set=(1 << 16) + 1)
+    assert e.value.code == C.ERR_ARG

@@ -189,3 +189,35 @@ def test_server_wait_timeout_and_cancel_before_launch():
         t2 = srv.submit_sign(ks, b"after", bytes(p.n))
         assert srv.wait(t2, key=ks)[0] == 0
         assert srv.cancel(t2) == C.ERR_ARG
+
+
+def test_server_remove_key_reuses_its_row():
+    """A removed key refuses new requests (-4) while its queued request still
+    completes; with one row per set, the next key can only be added once that
+    request has drained, and it reuses the row under a new id, which the old
+    id cannot reach."""
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(bytes(range(2 * 3 * p.n)), np.uint8))
+    with C.Server(hbm_budget_bytes=1 << 30, slots=1, max_keys_per_set=1) as srv:
+        k0 = srv.add_key(pid, bytes(sk[0]), bytes(pk[0]))
+        t = srv.submit_sign(k0, b"before removal", bytes(p.n))
+        srv.remove_key(k0)
+        with pytest.raises(C.CuspxError) as e:
+            srv.submit_sign(k0, b"after removal", bytes(p.n))
+        assert e.value.code == C.ERR_ARG
+        st, sig = srv.wait(t, key=k0)
+        assert st == 0
+        assert sig == O.signature_with_optrand(pid, b"before removal", bytes(sk[0]), bytes(p.n))
+        k1 = srv.add_key(pid, bytes(sk[1]), bytes(pk[1]))  # the drained row
+        assert k1 != k0
+        with pytest.raises(C.CuspxError):
+            srv.submit_sign(k0, b"stale id", bytes(p.n))
+        with pytest.raises(C.CuspxError):
+            srv.remove_key(k0)
+        st, sig = srv.wait(srv.submit_sign(k1, b"new key", bytes(p.n)), key=k1)
+        assert st == 0
+        assert sig == O.signature_with_optrand(pid, b"new key", bytes(sk[1]), bytes(p.n))
+        with pytest.raises(C.CuspxError) as e:  # no free row for a third key
+            srv.add_key(pid, bytes(sk[0]), bytes(pk[0]))
+        assert e.value.code == C.ERR_BUDGET
