@@ -11,7 +11,10 @@
 
 #include <dlfcn.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -141,17 +144,28 @@ int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 
 // ------------------------------------------------------------ the device
 
+// Large host-buffer transfers move in up to kChunks pieces: one per pinned
+// staging copy and one per PCIe copy, so the host copy of one piece overlaps
+// the DMA of the next.
+constexpr uint32_t kChunks = 32;
+
 // One stream of the host-buffer calls with its own device arena and pinned
-// staging (both grow-only).
+// staging (both grow-only), and the events that pace its chunked copies.
 struct Lane {
     hipStream_t stream = nullptr;
     void* dbuf = nullptr;
     size_t dcap = 0;
     uint8_t* hbuf = nullptr;
     size_t hcap = 0;
+    hipEvent_t ev[kChunks] = {};
+    hipEvent_t kdone = nullptr;  // after the slice's kernels, before its output copy
 
     int ensure(size_t dbytes, size_t hbytes) {
         if (!stream) HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (!kdone) {
+            for (auto& e : ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIP_OK(hipEventCreateWithFlags(&kdone, hipEventDisableTiming));
+        }
         if (dbytes > dcap) {
             if (dbuf) HIP_OK(hipFree(dbuf));
             dbuf = nullptr;
@@ -270,36 +284,112 @@ struct MsgPack {
     uint64_t bytes = 0;
 };
 
-// num items of `item` bytes from src (stride ss) to dst (stride ds).  Large
-// copies between the caller's buffers and the pinned staging are split over
-// host threads: one thread moves ~8-10 GB/s, so a 4096-signature batch
-// (70 MB) would otherwise spend longer in memcpy than on the GPU.
-static void copy_items(uint8_t* dst, uint64_t ds, const uint8_t* src, uint64_t ss, uint64_t item,
-                       uint32_t num) {
-    auto run = [=](uint32_t a, uint32_t b) {
-        if (ds == item && ss == item) {
-            memcpy(dst + a * item, src + a * item, (uint64_t)(b - a) * item);
+// Host threads that move data between the caller's buffers and the pinned
+// staging: one thread copies ~8-10 GB/s, so a 4096-signature batch (70 MB)
+// would otherwise spend longer in memcpy than on the GPU.  run(n, fn) calls
+// fn(0) .. fn(n-1), each once, on the workers and the calling thread; indices
+// are claimed in increasing order, and run returns when every call has.  The
+// pool lives until the process ends (never destroyed, so no thread is joined
+// during static destruction).
+class HostPool {
+  public:
+    static HostPool& get() {
+        static HostPool* p = new HostPool();
+        return *p;
+    }
+
+    void run(uint32_t n, const std::function<void(uint32_t)>& fn) {
+        if (n == 0) return;
+        if (n == 1 || workers_ == 0) {
+            for (uint32_t k = 0; k < n; k++) fn(k);
             return;
         }
-        for (uint32_t i = a; i < b; i++) memcpy(dst + i * ds, src + i * ss, item);
+        std::lock_guard<std::mutex> one(call_mu_);  // one batch at a time
+        Batch b{&fn, n};
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            cur_ = &b;
+            epoch_++;
+        }
+        cv_.notify_all();
+        drain(b);
+        std::unique_lock<std::mutex> lk(mu_);
+        cur_ = nullptr;  // late wakers find nothing; wait for those inside b
+        done_.wait(lk, [&] { return busy_ == 0; });
+    }
+
+  private:
+    struct Batch {
+        const std::function<void(uint32_t)>* fn;
+        uint32_t n;
+        std::atomic<uint32_t> next{0};
     };
-    const uint64_t total = item * num;
-    const unsigned hw = std::thread::hardware_concurrency();
-    unsigned t = (unsigned)(total >> 22);  // one thread per 4 MiB
-    t = t < 8 ? t : 8;
-    t = hw && t > hw ? hw : t;
-    if (t < 2) {
-        run(0, num);
+
+    HostPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        workers_ = hw > 1 ? (hw < 8 ? hw : 8) - 1 : 0;
+        for (unsigned i = 0; i < workers_; i++) std::thread([this] { loop(); }).detach();
+    }
+
+    static void drain(Batch& b) {
+        for (uint32_t k = b.next.fetch_add(1); k < b.n; k = b.next.fetch_add(1)) (*b.fn)(k);
+    }
+
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            Batch* b;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return cur_ && epoch_ != seen; });
+                seen = epoch_;
+                b = cur_;
+                busy_++;
+            }
+            drain(*b);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+
+    unsigned workers_ = 0;
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    Batch* cur_ = nullptr;
+    uint64_t epoch_ = 0;
+    unsigned busy_ = 0;
+};
+
+// Items [a, b) of `item` bytes from src (stride ss) to dst (stride ds).
+static void copy_rows(uint8_t* dst, uint64_t ds, const uint8_t* src, uint64_t ss, uint64_t item,
+                      uint32_t a, uint32_t b) {
+    if (ds == item && ss == item) {
+        memcpy(dst + a * item, src + a * item, (uint64_t)(b - a) * item);
         return;
     }
-    std::vector<std::thread> th;
-    const uint32_t per = (num + t - 1) / t;
-    for (unsigned k = 1; k < t; k++) {
-        const uint32_t a = k * per, b = a + per < num ? a + per : num;
-        if (a < b) th.emplace_back(run, a, b);
-    }
-    run(0, per < num ? per : num);
-    for (auto& x : th) x.join();
+    for (uint32_t i = a; i < b; i++) memcpy(dst + i * ds, src + i * ss, item);
+}
+
+// How a transfer of num items of `item` bytes is cut: pieces of at least
+// 1 MiB, at most kChunks of them; pieces(...).first = count, .second = items
+// per piece.
+static std::pair<uint32_t, uint32_t> pieces(uint32_t num, uint64_t item) {
+    const uint64_t total = item * num;
+    uint64_t c = total >> 20;
+    c = c < 1 ? 1 : c > kChunks ? kChunks : c;
+    const uint32_t per = (uint32_t)((num + c - 1) / c);
+    return {(num + per - 1) / per, per};
+}
+
+// num items of `item` bytes from src (stride ss) to dst (stride ds), split
+// over the host pool.
+static void copy_items(uint8_t* dst, uint64_t ds, const uint8_t* src, uint64_t ss, uint64_t item,
+                       uint32_t num) {
+    const auto pc = pieces(num, item);
+    HostPool::get().run(pc.first, [&](uint32_t c) {
+        const uint32_t a = c * pc.second, b = a + pc.second < num ? a + pc.second : num;
+        copy_rows(dst, ds, src, ss, item, a, b);
+    });
 }
 
 static MsgPack pack_msgs(uint32_t num, const uint32_t* mlen) {
@@ -325,17 +415,33 @@ struct Job {
     uint8_t* out = nullptr;  // sign: signatures; verify: statuses; keygen: pk
     uint64_t out_stride = 0;
     uint8_t* out2 = nullptr;  // keygen: sk
+    uint32_t pieces = 0, per = 0;  // sign: output copies, items per copy
+    hipEvent_t after = nullptr;    // chained slices: the previous slice's kernels
 };
 
 static int finish_job(Job& j) {
     if (!j.lane) return kOk;
     Lane& L = *j.lane;
     j.lane = nullptr;
-    HIP_OK(hipStreamSynchronize(L.stream));
     const SetOps* o = j.o;
     if (j.op == kOpSign) {
-        copy_items(j.out, j.out_stride, L.hbuf, o->sig_bytes, o->sig_bytes, j.num);
-    } else if (j.op == kOpVerify) {
+        // each piece is copied out as soon as its event says its DMA is done,
+        // while the later pieces are still in flight
+        std::atomic<int> err{kOk};
+        HostPool::get().run(j.pieces, [&](uint32_t c) {
+            if (hipEventSynchronize(L.ev[c]) != hipSuccess) {
+                err = kErrHip;
+                return;
+            }
+            const uint32_t a = c * j.per, b = a + j.per < j.num ? a + j.per : j.num;
+            copy_rows(j.out, j.out_stride, L.hbuf, o->sig_bytes, o->sig_bytes, a, b);
+        });
+        HIP_OK(hipStreamSynchronize(L.stream));
+        if (err != kOk) return fail(err, "output copy failed%s");
+        return kOk;
+    }
+    HIP_OK(hipStreamSynchronize(L.stream));
+    if (j.op == kOpVerify) {
         memcpy(j.out, L.hbuf, 4ull * j.num);
     } else {
         memcpy(j.out, L.hbuf, (uint64_t)j.num * o->pk_bytes);
@@ -347,14 +453,16 @@ static int finish_job(Job& j) {
 
 // Runs a host-buffer batch as `slices` consecutive slices on lanes 0..slices-1
 // (one slice: the whole batch on lane 0).  Slice c is staged and enqueued
-// while the slices before it already copy and compute; the slices' kernels
-// run concurrently on their streams (each persistent signing kernel fills
-// the CUs the others leave, each verify launch takes its own workgroups).
-// Every lane is finished at the end, in order, so a slice's outputs are
-// copied out while later slices still run.  enq(lane, first item, items,
+// while the slices before it already copy and compute.  Unchained, the
+// slices' kernels run concurrently on their streams (each verify launch takes
+// its own workgroups); chained, slice c's kernels start when slice c - 1's
+// have finished, so each slice's output copy overlaps the next slice's
+// kernels (signing, whose 17 KB-per-item output only exists once its launch
+// ends).  Every lane is finished at the end, in order, so a slice's outputs
+// are copied out while later slices still run.  enq(lane, first item, items,
 // device bytes allowed, job) enqueues one slice.
 template <class Enq>
-static int run_slices(uint32_t num, int slices, Enq enq) {
+static int run_slices(uint32_t num, int slices, Enq enq, bool chain = false) {
     if (num == 0) return kOk;
     Device& D = device();
     std::lock_guard<std::mutex> lk(D.mu);
@@ -367,6 +475,7 @@ static int run_slices(uint32_t num, int slices, Enq enq) {
     for (int c = 0; c < slices && err == kOk; c++) {
         const uint32_t a = c * per, b = a + per < num ? a + per : num;
         if (a >= b) break;
+        if (chain && c > 0) jobs[c].after = D.lane[c - 1].kdone;
         err = enq(D.lane[c], a, b - a, D.budget / slices, jobs[c]);
     }
     for (int c = 0; c < slices; c++) {
@@ -422,15 +531,27 @@ static int sign_enqueue(const SetOps* o, Lane& L, size_t budget, uint32_t num,
     b.sig = dsig;
     b.sig_stride = o->sig_bytes;
     b.ws = dsig + out_bytes;
+    if (job.after) HIP_OK(hipStreamWaitEvent(L.stream, job.after, 0));
     rc = o->sign(b, L.stream);
     if (rc) return fail(rc, "kernel launch failed%s");
-    HIP_OK(hipMemcpyAsync(h, dsig, (uint64_t)num * o->sig_bytes, hipMemcpyDeviceToHost, L.stream));
+    HIP_OK(hipEventRecord(L.kdone, L.stream));
+    const uint64_t sb = o->sig_bytes;
+    const auto pc = pieces(num, sb);
+    for (uint32_t c = 0; c < pc.first; c++) {
+        const uint64_t a = (uint64_t)c * pc.second;
+        const uint64_t cnt = a + pc.second < num ? pc.second : num - a;
+        HIP_OK(hipMemcpyAsync(h + a * sb, dsig + a * sb, cnt * sb, hipMemcpyDeviceToHost,
+                              L.stream));
+        HIP_OK(hipEventRecord(L.ev[c], L.stream));
+    }
     job.lane = &L;
     job.o = o;
     job.op = kOpSign;
     job.num = num;
     job.out = sig;
     job.out_stride = sig_stride;
+    job.pieces = pc.first;
+    job.per = pc.second;
     return kOk;
 }
 
@@ -444,7 +565,7 @@ static int sign_host(const SetOps* o, uint32_t num, const uint8_t* sk, uint64_t 
         return sign_enqueue(o, L, budget, cnt, sk + a * sk_stride, sk_stride, msg, moff + a,
                             mlen + a, optrand + (uint64_t)a * o->n, sig + a * sig_stride,
                             sig_stride, job);
-    });
+    }, /*chain=*/true);
 }
 
 static int verify_enqueue(const SetOps* o, Lane& L, size_t budget,
@@ -462,19 +583,35 @@ static int verify_enqueue(const SetOps* o, Lane& L, size_t budget,
     int rc = L.ensure(in_bytes + out_bytes + ws + 512, in_bytes);
     if (rc) return rc;
     uint8_t* h = L.hbuf;
+    uint8_t* d = (uint8_t*)L.dbuf;
     uint8_t* p = h;
     copy_items(p, o->pk_bytes, pk, pk_stride, o->pk_bytes, num);
-    p += al16((uint64_t)num * o->pk_bytes);
-    copy_items(p, o->sig_bytes, sig, sig_stride, o->sig_bytes, num);
-    p += al16((uint64_t)num * o->sig_bytes);
+    const uint64_t pk_region = al16((uint64_t)num * o->pk_bytes);
+    p += pk_region;
+    // the signatures (the bulk of the input): each piece goes to the device
+    // as soon as it is staged, while the pool stages the next ones
+    const uint64_t sb = o->sig_bytes;
+    const auto pc = pieces(num, sb);
+    std::atomic<int> err{kOk};
+    HostPool::get().run(pc.first, [&](uint32_t c) {
+        const uint32_t a = c * pc.second, e = a + pc.second < num ? a + pc.second : num;
+        copy_rows(p, sb, sig, sig_stride, sb, a, e);
+        if (hipMemcpyAsync(d + pk_region + a * sb, p + a * sb, (uint64_t)(e - a) * sb,
+                           hipMemcpyHostToDevice, L.stream) != hipSuccess)
+            err = kErrHip;
+    });
+    if (err != kOk) return fail(err, "input copy failed%s");
+    const uint64_t sig_region = al16((uint64_t)num * sb);
+    p += sig_region;
     memset(p, 0, mp.bytes);
     for (uint32_t i = 0; i < num; i++) memcpy(p + mp.off[i], msg + moff[i], mlen[i]);
     p += mp.bytes;
     memcpy(p, mp.off.data(), 8ull * num);
     p += al16(8ull * num);
     memcpy(p, mlen, 4ull * num);
-    uint8_t* d = (uint8_t*)L.dbuf;
-    HIP_OK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, L.stream));
+    const uint64_t tail = pk_region + sig_region;
+    HIP_OK(hipMemcpyAsync(d, h, pk_region, hipMemcpyHostToDevice, L.stream));
+    HIP_OK(hipMemcpyAsync(d + tail, h + tail, in_bytes - tail, hipMemcpyHostToDevice, L.stream));
     DevBatch b{};
     b.n = num;
     b.pk = d;
@@ -565,15 +702,18 @@ static int keygen_host(const SetOps* o, uint32_t num, const uint8_t* seeds, uint
     });
 }
 
-// slices of a multi-stream call: at least 2048 items each.  Smaller signing
-// launches leave part of the chip idle at their tails; a verify launch of
-// 1024 requests holds 64 workgroups for its whole 22-layer dependent path,
-// and four staggered ones measured slower than one (5.3 against 4.2 ms for
-// 4096 SHA2-128f items on host buffers).
+// slices of a multi-stream call: at most 2, of at least 2048 items (sign,
+// keygen) or 4096 (verify).  Measured on SHA2-128f host buffers, best of 3,
+// slices 1 / 2 / 3 / 4: chained signing slices at 4096 items 17.43 / 17.19 /
+// 17.17 / 17.09 ms and at 8192 33.80 / 32.23 / 32.98 / 32.84 ms (each slice's
+// output copy hides behind the next slice's kernels, smaller launches lose
+// at their tails); concurrent verify slices at 4096 3.60 / 5.02 / 5.39 /
+// 5.16 ms and at 8192 6.54 / 5.82 / 8.24 / 8.16 ms (a verify launch is bound
+// by its 22-layer dependent path, so it gains from sharing the CUs only when
+// each slice still fills them).
 static int ms_slices(int op, uint32_t num) {
-    (void)op;
-    const uint32_t s = num / 2048u;
-    return s < 1 ? 1 : s > (uint32_t)kLanes ? kLanes : (int)s;
+    const uint32_t s = num / (op == kOpVerify ? 4096u : 2048u);
+    return s < 1 ? 1 : s > 2 ? 2 : (int)s;
 }
 
 // draw the per-item optrand the way crypto_sign_signature does
@@ -590,6 +730,32 @@ static void draw_seed(uint8_t* out, int n) {
     drbg_bytes(out, 3ull * n);
     if (g_debug_mode)
         for (int i = 0; i < 3 * n; i++) out[i] = (uint8_t)i;
+}
+
+// Inputs of the batched calls, which the reference makes on the device
+// (dev_crypto_sign_keypair sign.cu:425-436, dev_crypto_sign_signature
+// sign.cu:2022-2027): in debug mode seeds 0..3n-1 and optrand 0xad with no
+// draw at all, so the host DRBG stream is left untouched; otherwise the
+// reference draws from a clock-seeded curand state per thread, and these
+// draw 3n / n host DRBG bytes per item instead (reproducible under
+// randombytes_init, and the same bytes as that many crypto_sign_keypair /
+// crypto_sign_signature calls).
+static void mdp_seeds(uint8_t* out, uint32_t num, int n) {
+    const uint64_t sb = 3ull * n;
+    for (uint32_t i = 0; i < num; i++) {
+        if (g_debug_mode)
+            for (uint64_t j = 0; j < sb; j++) out[i * sb + j] = (uint8_t)j;
+        else
+            drbg_bytes(out + i * sb, sb);
+    }
+}
+
+static void mdp_optrand(uint8_t* out, uint32_t num, int n) {
+    if (g_debug_mode) {
+        memset(out, 0xad, (uint64_t)num * n);
+        return;
+    }
+    for (uint32_t i = 0; i < num; i++) drbg_bytes(out + (uint64_t)i * n, n);
 }
 
 }  // namespace spx
@@ -759,7 +925,7 @@ static int mdp_keypair(unsigned char* pk, unsigned char* sk, unsigned int num, b
     const SetOps* o = get_set(default_pid());
     if (num && (!pk || !sk)) return fail(kErrArg, "null argument%s");
     std::vector<uint8_t> seeds(3ull * o->n * num);
-    for (unsigned i = 0; i < num; i++) draw_seed(seeds.data() + 3ull * o->n * i, o->n);
+    mdp_seeds(seeds.data(), num, o->n);
     return keygen_host(o, num, seeds.data(), 3ull * o->n, pk, sk,
                        ms ? ms_slices(kOpKeygen, num) : 1);
 }
@@ -776,7 +942,7 @@ static int mdp_sign(unsigned char* sm, unsigned long long* smlen, const unsigned
     std::vector<uint32_t> ml(num, (uint32_t)mlen);
     for (unsigned i = 0; i < num; i++) off[i] = mlen ? 32ull * i : 0;
     std::vector<uint8_t> opt((uint64_t)num * o->n);
-    draw_optrand(opt.data(), num, o->n);
+    mdp_optrand(opt.data(), num, o->n);
     static const uint8_t kNoMsg[1] = {0};
     int rc = sign_host(o, num, sk, o->sk_bytes, m ? m : kNoMsg, off.data(), ml.data(), opt.data(),
                        sm, stride, ms ? ms_slices(kOpSign, num) : 1);

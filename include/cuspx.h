@@ -14,9 +14,11 @@
  * or cuspx_set_default_param() is called).
  *
  * DEBUG_MODE: the reference is shipped with DEBUG_MODE defined
- * (all_option.h:74): keygen seeds become 0..3n-1 and optrand 0xad..ad after
- * the DRBG bytes were drawn.  The library starts in that mode;
- * cuspx_set_debug_mode(0) selects DRBG-derived seeds and optrand.
+ * (all_option.h:74): keygen seeds become 0..3n-1 and optrand 0xad..ad.  The
+ * single-item calls draw their DRBG bytes first and then replace them
+ * (sign.cu:342-349, 1950-1954); the batched face_mdp_* / face_ms_mdp_* calls
+ * draw nothing (sign.cu:425-436, 2022-2027).  The library starts in that
+ * mode; cuspx_set_debug_mode(0) selects DRBG-derived seeds and optrand.
  */
 #ifndef CUSPX_H
 #define CUSPX_H
@@ -90,8 +92,10 @@ int crypto_sign_open(unsigned char *m, unsigned long long *mlen, const unsigned 
 /* ------------------------------------- batched data-parallel surface (api.h) */
 
 /* replaces api.h:59 face_mdp_crypto_sign_keypair / sign.cu:1208.  num keys;
- * pk[num][2n], sk[num][4n].  Seeds: 0..3n-1 in debug mode (as the reference),
- * else 3n DRBG bytes per key in order (the reference draws curand). */
+ * pk[num][2n], sk[num][4n].  Seeds: 0..3n-1 in debug mode with no DRBG draw
+ * (as the reference), else 3n DRBG bytes per key in order (the reference
+ * draws curand).  face_mdp_crypto_sign's optrand likewise: 0xad..ad with no
+ * draw in debug mode, else n DRBG bytes per item in order. */
 int face_mdp_crypto_sign_keypair(unsigned char *pk, unsigned char *sk, unsigned int num);
 /* replaces api.h:107 face_mdp_crypto_sign / sign.cu:5098.  Message i is
  * m + 32*i (SPX_MLEN stride) of mlen <= 32 bytes (-4 otherwise); sm record i
@@ -109,10 +113,11 @@ int face_mdp_crypto_sign_open(unsigned char *m, unsigned long long *mlen, const 
  * sign.cu:1400, api.h:112 face_ms_mdp_crypto_sign / sign.cu:5305 and
  * api.h:151 face_ms_mdp_crypto_sign_open / sign.cu:7080 (the reference's
  * non-const pointers).  Same arguments, outputs and DRBG draws as the
- * face_mdp_* calls above; the batch is cut into up to 4 slices that run
- * concurrently on separate streams, so one slice's PCIe copies and host
- * staging overlap the other slices' kernels (slices of >= 2048 items;
- * smaller batches run as one slice). */
+ * face_mdp_* calls above; the batch is cut into up to 2 slices on separate
+ * streams (slices of >= 2048 items, >= 4096 for verification; smaller
+ * batches run as one slice).  Signing slices run back to back on the GPU, so
+ * one slice's output copy overlaps the next slice's kernels; verification
+ * and keygen slices run concurrently. */
 int face_ms_mdp_crypto_sign_keypair(unsigned char *pk, unsigned char *sk, unsigned int num);
 int face_ms_mdp_crypto_sign(unsigned char *sm, unsigned long long *smlen, unsigned char *m,
                             unsigned long long mlen, unsigned char *sk, unsigned int num);

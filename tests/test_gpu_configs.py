@@ -95,6 +95,26 @@ def test_config2_sha2_128f_kat_batch_4096():
                     d_mlen.ptr, d_st.ptr, ws.ptr, st.handle)
     st.sync()
     assert list(np.nonzero(d_st.download(np.int32))[0]) == bad
+    # the host-buffer calls on the same batch: the 70 MB of signatures leave
+    # (sign) and enter (verify) through pinned staging in pieces
+    print("config2: host-buffer sign + verify of %d" % B, flush=True)
+    sk = d_sk.download().reshape(B, p.sk_bytes)
+    hsig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    _compare("config2 host", [bytes(s) for s in hsig], _digests(g["sig"]))
+    assert list(np.nonzero(C.batch_verify(pid, pk, sig, msgs))[0]) == bad
+    # the reference's batched surface: signed-message records of SPX_BYTES +
+    # 32 bytes (the strided copy-out), one stream and four; optrand comes from
+    # the library's DRBG, so every record is checked by verification
+    prev = C.lib().cuspx_get_default_param()
+    C.set_default_param(pid)
+    try:
+        for ms in (False, True):
+            sm, _ = C.face_mdp_crypto_sign(np.frombuffer(b"".join(msgs), np.uint8), 32, sk, B,
+                                           ms=ms)
+            assert bytes(sm[:, p.sig_bytes:]) == b"".join(msgs)
+            assert np.count_nonzero(C.batch_verify(pid, pk, sm[:, :p.sig_bytes], msgs)) == 0
+    finally:
+        C.set_default_param(prev)
 
 
 def _keygen_items(names_seeds):
