@@ -2,11 +2,17 @@
 // generator the reference uses for keys, optrand and test messages
 // (rng.cu:116-241, PQCgenKAT_sign.cu:55-63).  The reference calls OpenSSL's
 // EVP AES-256-ECB; this file carries its own FIPS-197 AES-256 encryptor so
-// the library has no OpenSSL dependency.  State is process-global like the
+// the library has no OpenSSL dependency: the x86 AES instructions where the
+// CPU has them, else 32-bit round tables.  State is process-global like the
 // reference's DRBG_ctx (rng.cu:14) but guarded by a mutex.
+#include <atomic>
 #include <mutex>
 #include <stdint.h>
 #include <string.h>
+
+#if defined(__x86_64__)
+#include <immintrin.h>
+#endif
 
 #include "drbg.hpp"
 
@@ -94,16 +100,87 @@ struct Aes256 {
             out[4 * c + 3] = (uint8_t)o;
         }
     }
-    void encrypt(const uint8_t key[32], const uint8_t in[16], uint8_t out[16]) const {
-        uint32_t rk[60];
-        expand(key, rk);
-        encrypt_rk(rk, in, out);
-    }
 };
 
 const Aes256& aes() {
     static const Aes256 a;
     return a;
+}
+
+#if defined(__x86_64__)
+// AES-256 on the AES instructions.  The key schedule (FIPS-197 5.2) as 15
+// round-key blocks of 4 words: block k >= 2 is the running xor of block k - 2
+// (w[i] = w[i-8] ^ w[i-1] within a block) xored with one word broadcast to
+// all four lanes, SubWord(RotWord(w)) ^ Rcon of block k - 1's last word for
+// even k and SubWord of it for odd k.  AESKEYGENASSIST returns exactly
+// those two words in its lanes 3 and 2.
+#define SPX_AESNI __attribute__((target("aes,sse2")))
+
+SPX_AESNI static inline __m128i ks_next(__m128i two_back, __m128i word) {
+    __m128i x = two_back;
+    x = _mm_xor_si128(x, _mm_slli_si128(x, 4));
+    x = _mm_xor_si128(x, _mm_slli_si128(x, 4));
+    x = _mm_xor_si128(x, _mm_slli_si128(x, 4));
+    return _mm_xor_si128(x, word);
+}
+
+template <int RCON>
+SPX_AESNI static inline void ks_pair(__m128i* rk, int k) {
+    rk[k] = ks_next(rk[k - 2], _mm_shuffle_epi32(_mm_aeskeygenassist_si128(rk[k - 1], RCON), 0xff));
+    if (k + 1 < 15)
+        rk[k + 1] = ks_next(rk[k - 1],
+                            _mm_shuffle_epi32(_mm_aeskeygenassist_si128(rk[k], 0), 0xaa));
+}
+
+SPX_AESNI static void ni_expand(const uint8_t key[32], uint32_t w[60]) {
+    __m128i rk[15];
+    rk[0] = _mm_loadu_si128((const __m128i*)key);
+    rk[1] = _mm_loadu_si128((const __m128i*)(key + 16));
+    ks_pair<0x01>(rk, 2);
+    ks_pair<0x02>(rk, 4);
+    ks_pair<0x04>(rk, 6);
+    ks_pair<0x08>(rk, 8);
+    ks_pair<0x10>(rk, 10);
+    ks_pair<0x20>(rk, 12);
+    ks_pair<0x40>(rk, 14);
+    memcpy(w, rk, sizeof rk);
+}
+
+SPX_AESNI static void ni_encrypt(const uint32_t w[60], const uint8_t in[16], uint8_t out[16]) {
+    __m128i rk[15];
+    memcpy(rk, w, sizeof rk);
+    __m128i x = _mm_xor_si128(_mm_loadu_si128((const __m128i*)in), rk[0]);
+    for (int r = 1; r < 14; r++) x = _mm_aesenc_si128(x, rk[r]);
+    _mm_storeu_si128((__m128i*)out, _mm_aesenclast_si128(x, rk[14]));
+}
+
+static bool cpu_has_aes() { return __builtin_cpu_supports("aes"); }
+#else
+static void ni_expand(const uint8_t*, uint32_t*) {}
+static void ni_encrypt(const uint32_t*, const uint8_t*, uint8_t*) {}
+static bool cpu_has_aes() { return false; }
+#endif
+
+std::atomic<bool> g_portable{false};
+
+// A key schedule in the form of the encryptor that made it (the AES
+// instructions' 15 blocks of key bytes, or the round tables' row-0-first
+// words); both are 60 words.
+struct Sched {
+    alignas(16) uint32_t w[60];
+    bool ni;
+};
+
+void expand(const uint8_t key[32], Sched& s) {
+    static const bool has = cpu_has_aes();
+    s.ni = has && !g_portable.load(std::memory_order_relaxed);
+    if (s.ni) ni_expand(key, s.w);
+    else aes().expand(key, s.w);
+}
+
+void encrypt(const Sched& s, const uint8_t in[16], uint8_t out[16]) {
+    if (s.ni) ni_encrypt(s.w, in, out);
+    else aes().encrypt_rk(s.w, in, out);
 }
 
 struct Drbg {
@@ -122,11 +199,11 @@ struct Drbg {
         }
     }
     // AES_CTR_DRBG_Update (rng.cu:217-241) with the current key's schedule
-    void update(const uint32_t rk[60], const uint8_t* provided) {
+    void update(const Sched& rk, const uint8_t* provided) {
         uint8_t temp[48];
         for (int i = 0; i < 3; i++) {
             inc(v);
-            aes().encrypt_rk(rk, v, temp + 16 * i);
+            encrypt(rk, v, temp + 16 * i);
         }
         if (provided)
             for (int i = 0; i < 48; i++) temp[i] ^= provided[i];
@@ -143,8 +220,12 @@ Drbg& drbg() {
 }  // namespace
 
 void aes256_ecb(const uint8_t key[32], const uint8_t in[16], uint8_t out[16]) {
-    aes().encrypt(key, in, out);
+    Sched s;
+    expand(key, s);
+    encrypt(s, in, out);
 }
+
+void aes_force_portable(bool on) { g_portable = on; }
 
 void drbg_init(const uint8_t entropy[48], const uint8_t* pers) {
     Drbg& d = drbg();
@@ -155,8 +236,8 @@ void drbg_init(const uint8_t entropy[48], const uint8_t* pers) {
         for (int i = 0; i < 48; i++) seed[i] ^= pers[i];
     memset(d.key, 0, 32);
     memset(d.v, 0, 16);
-    uint32_t rk[60];
-    aes().expand(d.key, rk);
+    Sched rk;
+    expand(d.key, rk);
     d.update(rk, seed);
     d.reseed_counter = 1;
 }
@@ -165,12 +246,12 @@ void drbg_bytes(uint8_t* x, uint64_t xlen) {
     Drbg& d = drbg();
     std::lock_guard<std::mutex> lk(d.mu);
     uint8_t block[16];
-    uint32_t rk[60];  // the key is fixed until the update below: expand it once
-    aes().expand(d.key, rk);
+    Sched rk;  // the key is fixed until the update below: expand it once
+    expand(d.key, rk);
     uint64_t i = 0;
     while (xlen > 0) {
         Drbg::inc(d.v);
-        aes().encrypt_rk(rk, d.v, block);
+        encrypt(rk, d.v, block);
         const uint64_t take = xlen > 15 ? 16 : xlen;
         memcpy(x + i, block, take);
         i += take;

@@ -1,7 +1,9 @@
 """The library's DRBG and AES-256 (cuspx_amd/csrc/drbg.cpp) built natively
 with g++ under AddressSanitizer / UndefinedBehaviorSanitizer and checked
 against the FIPS-197 Appendix C.3 AES-256 example and the NIST PQC KAT
-DRBG's first output (entropy 00..2f, the rng.cu:139-241 construction)."""
+DRBG's first output (entropy 00..2f, the rng.cu:139-241 construction), with
+the AES instructions and with the round tables, which must also agree on
+1 MiB of draws of every short length."""
 import os
 import shutil
 import subprocess
@@ -14,24 +16,33 @@ CSRC = os.path.join(ROOT, "cuspx_amd", "csrc")
 HARNESS = r"""
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include "drbg.hpp"
+// each line: the DRBG's first 48 bytes, then the FIPS-197 C.3 ciphertext, once
+// per AES implementation (the AES instructions where the CPU has them, then
+// the round tables); last line: whether 1 MiB of mixed-length draws agree
+static uint8_t big[2][1 << 20];
 int main() {
-    uint8_t e[48];
-    for (int i = 0; i < 48; i++) e[i] = (uint8_t)i;
-    spx::drbg_init(e, nullptr);
-    uint8_t out[48];
-    spx::drbg_bytes(out, 48);
-    for (int i = 0; i < 48; i++) printf("%02X", out[i]);
-    printf("\n");
-    static uint8_t big[1 << 16];
-    spx::drbg_bytes(big, sizeof big);             // long draws and every short length
-    for (int n = 0; n < 70; n++) spx::drbg_bytes(big, (uint64_t)n);
-    uint8_t k[32], in[16], o[16];
-    for (int i = 0; i < 32; i++) k[i] = (uint8_t)i;
-    for (int i = 0; i < 16; i++) in[i] = (uint8_t)(0x11 * i);
-    spx::aes256_ecb(k, in, o);
-    for (int i = 0; i < 16; i++) printf("%02x", o[i]);
-    printf("\n");
+    for (int mode = 0; mode < 2; mode++) {
+        spx::aes_force_portable(mode == 1);
+        uint8_t e[48];
+        for (int i = 0; i < 48; i++) e[i] = (uint8_t)i;
+        spx::drbg_init(e, nullptr);
+        uint8_t out[48];
+        spx::drbg_bytes(out, 48);
+        for (int i = 0; i < 48; i++) printf("%02X", out[i]);
+        printf("\n");
+        uint64_t off = 0;                             // every short length, then a long draw
+        for (int n = 0; n < 70; n++, off += (uint64_t)n - 1) spx::drbg_bytes(big[mode] + off, n);
+        spx::drbg_bytes(big[mode] + off, sizeof big[mode] - off);
+        uint8_t k[32], in[16], o[16];
+        for (int i = 0; i < 32; i++) k[i] = (uint8_t)i;
+        for (int i = 0; i < 16; i++) in[i] = (uint8_t)(0x11 * i);
+        spx::aes256_ecb(k, in, o);
+        for (int i = 0; i < 16; i++) printf("%02x", o[i]);
+        printf("\n");
+    }
+    printf("%s\n", memcmp(big[0], big[1], sizeof big[0]) == 0 ? "same" : "different");
     return 0;
 }
 """
@@ -50,7 +61,10 @@ def test_drbg_and_aes_known_answers_under_sanitizers(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.split()
-    assert lines[0] == ("061550234D158C5EC95595FE04EF7A25767F2E24CC2BC479D09D86DC9ABCFDE7"
-                        "056A8C266F9EF97ED08541DBD2E1FFA1")
-    assert lines[1] == "8ea2b7ca516745bfeafc49904b496089"  # FIPS-197 C.3
+    kat = ("061550234D158C5EC95595FE04EF7A25767F2E24CC2BC479D09D86DC9ABCFDE7"
+           "056A8C266F9EF97ED08541DBD2E1FFA1")
+    for mode in range(2):  # AES instructions (or tables without them), tables
+        assert lines[2 * mode] == kat
+        assert lines[2 * mode + 1] == "8ea2b7ca516745bfeafc49904b496089"  # FIPS-197 C.3
+    assert lines[4] == "same"
     assert "runtime error" not in r.stderr
