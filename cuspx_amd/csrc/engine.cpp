@@ -282,6 +282,7 @@ void prof_mark(hipStream_t s, const char* kernel) {
 struct MsgPack {
     std::vector<uint64_t> off;  // device offsets, 16-byte aligned
     uint64_t bytes = 0;
+    uint32_t longest = 0;
 };
 
 // Host threads that move data between the caller's buffers and the pinned
@@ -399,6 +400,7 @@ static MsgPack pack_msgs(uint32_t num, const uint32_t* mlen) {
     for (uint32_t i = 0; i < num; i++) {
         mp.off[i] = o;
         o += al16(mlen[i]);
+        mp.longest = mlen[i] > mp.longest ? mlen[i] : mp.longest;
     }
     mp.bytes = o + 16;
     return mp;
@@ -525,6 +527,7 @@ static int sign_enqueue(const SetOps* o, Lane& L, size_t budget, uint32_t num,
     b.moff = (const uint64_t*)q;
     q += al16(8ull * num);
     b.mlen = (const uint32_t*)q;
+    b.mlen_bound = mp.longest > 0 ? mp.longest : 1;
     q += al16(4ull * num);
     b.optrand = q;
     uint8_t* dsig = d + in_bytes;
@@ -625,6 +628,7 @@ static int verify_enqueue(const SetOps* o, Lane& L, size_t budget,
     b.moff = (const uint64_t*)q;
     q += al16(8ull * num);
     b.mlen = (const uint32_t*)q;
+    b.mlen_bound = mp.longest > 0 ? mp.longest : 1;
     int32_t* dst = (int32_t*)(d + in_bytes);
     b.status = dst;
     b.ws = (uint8_t*)dst + out_bytes;

@@ -31,6 +31,7 @@ struct DevBatch {
     const uint8_t* msg;      // msg + moff[i]: 4-byte aligned, readable to a 4-byte boundary
     const uint64_t* moff;
     const uint32_t* mlen;
+    uint32_t mlen_bound;     // no message is longer (0: not known)
     const uint8_t* optrand;  // [n][param n]
     const uint32_t* key_idx; // optional key row per request (sk / pk tables)
     const uint32_t* key_mid; // optional precomputed midstate per key row (8 words)

@@ -132,12 +132,12 @@ struct FamilyImpl {
         T.ctr = n0 ? A0.tile_ctr : n1 ? A1.tile_ctr : A2.tile_ctr;
         (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, "family");
-        if (n0) hipLaunchKernelGGL((k_prep<P0>), dim3(cdiv((uint64_t)n0 * kPrepLanes, 64)),
-                                   dim3(64), 0, s, A0);
-        if (n1) hipLaunchKernelGGL((k_prep<P1>), dim3(cdiv((uint64_t)n1 * kPrepLanes, 64)),
-                                   dim3(64), 0, s, A1);
-        if (n2) hipLaunchKernelGGL((k_prep<P2>), dim3(cdiv((uint64_t)n2 * kPrepLanes, 64)),
-                                   dim3(64), 0, s, A2);
+        if (n0) hipLaunchKernelGGL((k_prep<P0>), dim3(prep_grid<P0>(A0, n0, b[0].mlen_bound)),
+                                   dim3(kPrepWg), 0, s, A0);
+        if (n1) hipLaunchKernelGGL((k_prep<P1>), dim3(prep_grid<P1>(A1, n1, b[1].mlen_bound)),
+                                   dim3(kPrepWg), 0, s, A1);
+        if (n2) hipLaunchKernelGGL((k_prep<P2>), dim3(prep_grid<P2>(A2, n2, b[2].mlen_bound)),
+                                   dim3(kPrepWg), 0, s, A2);
         prof_mark(s, "k_prep");
         static const uint32_t resident = [] {
             int per_cu = 0, cus = 0, dev = 0;

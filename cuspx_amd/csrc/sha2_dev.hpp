@@ -129,31 +129,86 @@ __device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32
     s[3] += q3;
 }
 
+// a word pointer into LDS (address space 3), so that code reached through a
+// call still accesses it with ds_* instructions rather than flat ones
+typedef uint32_t __attribute__((address_space(3))) lds_u32;
+// a word pointer into global memory (address space 1): global_load rather
+// than flat, so message loads are not counted with LDS accesses
+typedef const uint32_t __attribute__((address_space(1))) gbl_u32;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef u32x4 __attribute__((address_space(3))) lds_u32x4;
+
 // the 64 rounds of sha256_compress_pair with K[t] + W[t] precomputed (kw,
 // read by both lanes; the odd lane masks it out)
-__device__ __forceinline__ void sha256_rounds_pair_kw(uint32_t s[4], const uint32_t* kw) {
+// The 64 rounds of a lane pair with K[t] + W[t] precomputed (kw, read by both
+// lanes), with the odd (a-side) lane two rounds behind the even (e-side) one.
+// In sha256_compress_pair every round waits for its exchange: e' = d + T1
+// needs the odd lane's d and a' = T1 + T2 the even lane's T1, so each round
+// is alignbit, xor3, add3, select, DPP swap and add in a row.  Here each
+// lane's own recurrence is alignbit, xor3, add3:
+//   even, iteration i:  E[i+1] = S1(E[i]) + Ch(E[i], E[i-1], E[i-2]) + pre[i],
+//                       pre[i] = E[i-3] + KW[i] + A[i-3]
+//   odd,  iteration i:  A[i-1] = S0(A[i-2]) + Maj(A[i-2], A[i-3], A[i-4]) + T1[i-2],
+//                       T1[i-2] = E[i-1] - A[i-5]
+// and the swap of iteration i (each lane sends its new value) is first used
+// in iteration i + 1, to form pre[i+1] (even) and T1[i-1] (odd).  66
+// iterations; the odd lane's outputs of the first two are b and a, the even
+// lane's state is taken after iteration 63.  K + W is read about 16 rounds
+// ahead of its use (two groups of 16 registers).
+template <class KW>
+__device__ __forceinline__ void sha256_rounds_pair_kw(uint32_t s[4], KW kw) {
     const bool odd = threadIdx.x & 1u;
     const uint32_t m = odd ? ~0u : 0u;
     const uint32_t r1 = odd ? 2u : 6u, r2 = odd ? 13u : 11u, r3 = odd ? 22u : 25u;
-    uint32_t q0 = s[0], q1 = s[1], q2 = s[2], q3 = s[3];
+    uint32_t c0[16], c1[16];  // K + W [16g, 16g + 16) and the group after it
 #pragma unroll
-    for (int t = 0; t < 64; t++) {
-        const uint32_t hk = (q3 + kw[t]) & ~m;
+    for (int j = 0; j < 16; j++) c0[j] = kw[j];
+#pragma unroll
+    for (int j = 0; j < 16; j++) c1[j] = kw[16 + j];
+    // even: e f g h; odd: a b c d.  The odd lane starts at c, d; the even
+    // lane needs d (pre[0]) and c (A[-2], for pre[1]) from it.
+    const uint32_t d_ev = pair_swap(s[3]), c_ev = pair_swap(s[2]);
+    const uint32_t a_od = s[0], b_od = s[1];
+    uint32_t q0 = lane_sel(m, s[2], s[0]), q1 = lane_sel(m, s[3], s[1]), q2 = s[2], q3 = s[3];
+    uint32_t x = q3 + c0[0] + d_ev;  // even: pre[0]; odd: unused
+    uint32_t rp = c_ev;              // the previous iteration's swap
+    uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+#pragma unroll
+    for (int i = 0; i < 66; i++) {
+        if (i == 15 || i == 31 || i == 47) {  // next group of K + W
+#pragma unroll
+            for (int j = 0; j < 16; j++) c0[j] = c1[j];
+            if (i < 47) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) c1[j] = kw[i + 17 + j];
+            }
+        }
         const uint32_t sg = xor3(__builtin_amdgcn_alignbit(q0, q0, r1),
                                  __builtin_amdgcn_alignbit(q0, q0, r2),
                                  __builtin_amdgcn_alignbit(q0, q0, r3));
-        const uint32_t fn = bit_ch(bit_chi(q0, q1, m), q1, q2);
-        const uint32_t tt = sg + fn + hk;
-        const uint32_t nw = tt + pair_swap(lane_sel(m, q3, tt));
+        const uint32_t fn = bit_ch(bit_chi(q0, q1, m), q1, q2);  // Ch | Maj
+        uint32_t nw = sg + fn + x;                                // E[i+1] | A[i-1]
+        if (i == 0) nw = lane_sel(m, b_od, nw);
+        if (i == 1) nw = lane_sel(m, a_od, nw);
+        const uint32_t r = pair_swap(nw);
+        const uint32_t kn = i + 1 < 64 ? c0[(i + 1) & 15] : 0u;
+        x = lane_sel(m, rp - q2, q2 + kn + rp);  // T1[i-1] | pre[i+1]
         q3 = q2;
         q2 = q1;
         q1 = q0;
         q0 = nw;
+        rp = r;
+        if (i == 63) {
+            e0 = q0;
+            e1 = q1;
+            e2 = q2;
+            e3 = q3;
+        }
     }
-    s[0] += q0;
-    s[1] += q1;
-    s[2] += q2;
-    s[3] += q3;
+    s[0] += lane_sel(m, q0, e0);
+    s[1] += lane_sel(m, q1, e1);
+    s[2] += lane_sel(m, q2, e2);
+    s[3] += lane_sel(m, q3, e3);
 }
 
 // sha256_compress on a lane pair whose lanes both hold the whole chaining
@@ -172,6 +227,47 @@ __device__ __forceinline__ void sha256_compress_dup(uint32_t st[8], const uint32
         const uint32_t other = pair_swap(s4[i]);
         st[i] = lane_sel(m, s4[i], other);
         st[4 + i] = lane_sel(m, other, s4[i]);
+    }
+}
+
+// sha256_compress_dup with K[t] + W[t] precomputed (kw, LDS): the message
+// hashes' blocks made only of message words, whose schedules another wave
+// computed ahead
+__device__ __forceinline__ void sha256_rounds_dup_kw(uint32_t st[8], const lds_u32* kw) {
+    const uint32_t m = (threadIdx.x & 1u) ? ~0u : 0u;
+    uint32_t s4[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) s4[i] = lane_sel(m, st[i], st[4 + i]);
+    sha256_rounds_pair_kw(s4, kw);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t other = pair_swap(s4[i]);
+        st[i] = lane_sel(m, s4[i], other);
+        st[4 + i] = lane_sel(m, other, s4[i]);
+    }
+}
+
+// K[t] + W[t], t < 64, of the block whose words are bswap32(src[0..16))
+// (a block made only of message words) into kw[0..64), 16-byte aligned
+__device__ __forceinline__ void sha256_kw_msg_block(const uint32_t* src, lds_u32* kw) {
+    const gbl_u32* g = (gbl_u32*)src;  // address-space cast
+    uint32_t w[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) w[i] = bswap32(g[i]);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int t = 4 * q + j;
+            if (t >= 16) {
+                const uint32_t x = w[(t - 15) & 15], y = w[(t - 2) & 15];
+                w[t & 15] = xor3(rotr32(y, 17), rotr32(y, 19), y >> 10) + w[(t - 7) & 15] +
+                            xor3(rotr32(x, 7), rotr32(x, 18), x >> 3) + w[t & 15];
+            }
+            o[j] = kSha256K[t] + w[t & 15];
+        }
+        *reinterpret_cast<lds_u32x4*>(kw + 4 * q) = u32x4{o[0], o[1], o[2], o[3]};
     }
 }
 
@@ -352,28 +448,27 @@ struct Sha2 {
     // mlen are never read beyond the word holding the last byte.  Blocks made
     // only of whole message words are loaded directly; the head (prefix) and
     // the tail (last partial word, padding, length) go word by word.
+    //
+    // stream_range runs the words [kk, kk_end) of that stream (kk a multiple
+    // of 16; kk_end clipped to the padded end), stream_msg all of them.
     template <int NPRE>
-    __device__ static void stream_msg(uint32_t st[8], const uint32_t pre[NPRE],
-                                      const uint8_t* m, uint32_t mlen, uint32_t before) {
+    __device__ static void stream_range(uint32_t st[8], const uint32_t pre[NPRE],
+                                        const uint8_t* m, uint32_t mlen, uint32_t before,
+                                        uint32_t kk, uint32_t kk_end) {
         const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
         const uint32_t ndata = NPRE + (mlen + 4) / 4;  // words holding data or 0x80
         const uint32_t nfull = NPRE + mlen / 4;        // words made of message bytes only
         const uint32_t total = ((ndata + 2 + 15) / 16) * 16;
-        const uint32_t* mw = reinterpret_cast<const uint32_t*>(m);
-        uint32_t b[16];
+        if (kk_end > total) kk_end = total;
+        (void)nfull;
+        const gbl_u32* mw = (gbl_u32*)m;  // address-space cast
+        uint32_t sl[8], b[16];  // the chaining value stays in registers
+#pragma unroll
+        for (int i = 0; i < 8; i++) sl[i] = st[i];
 #pragma unroll
         for (int i = 0; i < 16; i++) b[i] = 0;
-        uint32_t kk = 0;
 #pragma unroll 1
-        while (kk < total) {
-            if ((kk & 15) == 0 && kk >= (uint32_t)NPRE && kk + 16 <= nfull) {
-                const uint32_t* src = mw + (kk - NPRE);
-#pragma unroll
-                for (int i = 0; i < 16; i++) b[i] = bswap32(src[i]);
-                sha256_compress_dup(st, b);
-                kk += 16;
-                continue;
-            }
+        while (kk < kk_end) {
             uint32_t x;
             if (kk < (uint32_t)NPRE) {
                 x = pre[0];
@@ -397,21 +492,122 @@ struct Sha2 {
                 x = 0;
             }
             put16(b, (int)(kk & 15), x);
-            if ((kk & 15) == 15) sha256_compress_dup(st, b);
+            if ((kk & 15) == 15) sha256_compress_dup(sl, b);
             kk++;
         }
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = sl[i];
     }
 
-    // R = HMAC-SHA256(sk_prf, optrand || m)[:n]  (hash_sha256.cu:99-144)
+    // nfb blocks made only of message words, block b = src[16 b .. 16 b + 16),
+    // on the lane pair; block b + 1 is loaded while block b compresses
+    __device__ static void stream_full(uint32_t st[8], const uint32_t* src, uint32_t nfb) {
+        const gbl_u32* g = (gbl_u32*)src;  // address-space cast
+        uint32_t sl[8], cur[16], nxt[16];
+#pragma unroll
+        for (int i = 0; i < 8; i++) sl[i] = st[i];
+#pragma unroll
+        for (int i = 0; i < 16; i++) cur[i] = g[i];
+#pragma unroll 1
+        for (uint32_t b = 0; b < nfb; b++) {
+            if (b + 1 < nfb) {
+#pragma unroll
+                for (int i = 0; i < 16; i++) nxt[i] = g[16 * (b + 1) + i];
+            }
+            uint32_t w[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) w[i] = bswap32(cur[i]);
+            sha256_compress_dup(sl, w);
+#pragma unroll
+            for (int i = 0; i < 16; i++) cur[i] = nxt[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = sl[i];
+    }
+
+    // the whole stream on the lane pair: head words, the blocks made only of
+    // message words (from the first block boundary after the prefix), tail
+    template <int NPRE>
+    __device__ static void stream_msg(uint32_t st[8], const uint32_t pre[NPRE],
+                                      const uint8_t* m, uint32_t mlen, uint32_t before) {
+        constexpr uint32_t kk0 = (NPRE + 15) / 16 * 16;
+        const uint32_t nfull = NPRE + mlen / 4;
+        const uint32_t nfb = nfull >= kk0 + 16 ? (nfull - kk0) / 16 : 0;
+        if (!nfb) {
+            stream_range<NPRE>(st, pre, m, mlen, before, 0, ~0u);
+            return;
+        }
+        stream_range<NPRE>(st, pre, m, mlen, before, 0, kk0);
+        stream_full(st, reinterpret_cast<const uint32_t*>(m) + (kk0 - NPRE), nfb);
+        stream_range<NPRE>(st, pre, m, mlen, before, kk0 + 16 * nfb, ~0u);
+    }
+
+    // The same stream run by a two-wave workgroup (long messages).  The
+    // blocks made only of message words (from the first block boundary
+    // after the prefix) have schedules that do not depend on the chaining
+    // value: wave 1 (prod) computes their K + W, kLongStage blocks per stage
+    // (one per lane) into one half of kw_s, while wave 0's lane pairs run
+    // the rounds of the previous stage from the other half.  Head and tail
+    // blocks go through stream_range.  Both waves call this with the same
+    // arguments and pass the same 1 + stages barriers.
+    static constexpr uint32_t kLongStage = 32;
+    static constexpr uint32_t kLongLdsWords = 2 * kLongStage * 64;
+
+    template <int NPRE>
+    __device__ static void stream_msg_long(uint32_t st[8], const uint32_t pre[NPRE],
+                                           const uint8_t* m, uint32_t mlen, uint32_t before,
+                                           lds_u32* kw_s, bool prod) {
+        constexpr uint32_t kk0 = (NPRE + 15) / 16 * 16;  // first block boundary after pre
+        const uint32_t nfull = NPRE + mlen / 4;
+        const uint32_t nfb = nfull >= kk0 + 16 ? (nfull - kk0) / 16 : 0;
+        const uint32_t nst = (nfb + kLongStage - 1) / kLongStage;
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t* mw = reinterpret_cast<const uint32_t*>(m) + (kk0 - NPRE);
+        auto produce = [&](uint32_t c) {
+            const uint32_t blk = c * kLongStage + lane;
+            if (lane < kLongStage && blk < nfb)
+                sha256_kw_msg_block(mw + 16 * blk, kw_s + ((c & 1) * kLongStage + lane) * 64);
+        };
+        if (prod) produce(0);
+        else if (nfb) stream_range<NPRE>(st, pre, m, mlen, before, 0, kk0);
+        __syncthreads();
+        uint32_t sl[8];  // in registers across the stages
+#pragma unroll
+        for (int i = 0; i < 8; i++) sl[i] = st[i];
+#pragma unroll 1
+        for (uint32_t c = 0; c < nst; c++) {
+            if (prod) {
+                if (c + 1 < nst) produce(c + 1);
+            } else {
+                const uint32_t nb = nfb - c * kLongStage < kLongStage ? nfb - c * kLongStage
+                                                                      : kLongStage;
+                const lds_u32* rows = kw_s + (c & 1) * kLongStage * 64;
+#pragma unroll 1
+                for (uint32_t b = 0; b < nb; b++) sha256_rounds_dup_kw(sl, rows + 64 * b);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) st[i] = sl[i];
+        if (!prod) stream_range<NPRE>(st, pre, m, mlen, before, nfb ? kk0 + 16 * nfb : 0, ~0u);
+    }
+
+    // R = HMAC-SHA256(sk_prf, optrand || m)[:n]  (hash_sha256.cu:99-144).
+    // LONG: the two-wave long-message form (stream_msg_long over kw_s; prod =
+    // this thread is in the producing wave), else one lane pair.  (The form
+    // is a template argument: kw_s, an LDS address, may well be 0.)
+    template <bool LONG = false>
     __device__ static void prf_msg(const uint32_t skprf[NW], const uint32_t opt[NW],
-                                   const uint8_t* m, uint32_t mlen, uint32_t R[NW]) {
+                                   const uint8_t* m, uint32_t mlen, uint32_t R[NW],
+                                   lds_u32* kw_s = nullptr, bool prod = false) {
         uint32_t st[8], w[16], inner[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
 #pragma unroll
         for (int i = 0; i < 16; i++) w[i] = (i < NW ? skprf[i] : 0u) ^ 0x36363636u;
         sha256_compress_dup(st, w);
-        stream_msg<NW>(st, opt, m, mlen, 64);
+        if constexpr (LONG) stream_msg_long<NW>(st, opt, m, mlen, 64, kw_s, prod);
+        else stream_msg<NW>(st, opt, m, mlen, 64);
 #pragma unroll
         for (int i = 0; i < 8; i++) inner[i] = st[i];
 #pragma unroll
@@ -428,9 +624,10 @@ struct Sha2 {
     }
 
     // digest bytes = MGF1(SHA256(R || pk || m), DG)  (hash_sha256.cu:200-247)
-    template <int DG>
+    template <int DG, bool LONG = false>
     __device__ static void h_msg(const uint32_t R[NW], const uint32_t pk[2 * NW],
-                                 const uint8_t* m, uint32_t mlen, uint8_t dg[DG]) {
+                                 const uint8_t* m, uint32_t mlen, uint8_t dg[DG],
+                                 lds_u32* kw_s = nullptr, bool prod = false) {
         uint32_t pre[3 * NW], st[8], seed[8], w[16];
 #pragma unroll
         for (int i = 0; i < NW; i++) pre[i] = R[i];
@@ -438,7 +635,8 @@ struct Sha2 {
         for (int i = 0; i < 2 * NW; i++) pre[NW + i] = pk[i];
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = kSha256IV[i];
-        stream_msg<3 * NW>(st, pre, m, mlen, 0);
+        if constexpr (LONG) stream_msg_long<3 * NW>(st, pre, m, mlen, 0, kw_s, prod);
+        else stream_msg<3 * NW>(st, pre, m, mlen, 0);
 #pragma unroll
         for (int i = 0; i < 8; i++) seed[i] = st[i];
 #pragma unroll

@@ -68,6 +68,10 @@ struct BatchArgs {
     uint32_t* tile_ctr;       // fused signing: next tile (zeroed by the prep kernel)
     uint32_t* fors_count;     // FORS roots finished per request (zeroed by the prep kernels)
     uint32_t n_req;
+    // prep kernels: workgroups of the lane-pair part; messages of long_min
+    // bytes or more (0: none) are hashed by the two-wave part after them
+    uint32_t prep_wgs;
+    uint32_t long_min;
 };
 
 // ----------------------------------------------------- message / indices
@@ -100,22 +104,48 @@ __device__ __forceinline__ void plan_from_digest(Plan& pl, const uint8_t* dg) {
     pl.leaf = leaf;
 }
 
-// lanes per request of the prep kernels: every request hashes its message
-// on a lane pair (sha256_compress_dup / Shake::stream_msg_pair; both lanes
-// compute everything, the even lane stores)
+// The prep kernels (128-lane workgroups).  Workgroups [0, prep_wgs): every
+// request hashes its message on a lane pair (sha256_compress_dup /
+// Shake::stream_msg_pair; both lanes compute everything, the even lane
+// stores), 64 requests per workgroup.  SHA-256 messages of at least
+// long_min bytes are left to the workgroups after those: each takes long
+// requests in turn and runs one with both waves (Sha2::stream_msg_long), the
+// schedules of its message-only blocks computed by wave 1 ahead of wave 0's
+// rounds.
 constexpr uint32_t kPrepLanes = 2;
+constexpr uint32_t kPrepWg = 128;
+// the two-wave path from this length: a single SHA2-128f / 256f request of
+// 512 B signs in 0.329 / 0.624 ms against 0.336 / 0.634 on the lane pair,
+// 2 KiB in 0.396 / 0.686 against 0.432 / 0.723, 64 KiB in 3.28 / 3.56
+// against 4.60 / 5.07 (below 512 B the two are equal)
+constexpr uint32_t kLongMsgBytes = 512;
+constexpr uint32_t kLongWgs = 512;         // at most this many two-wave workgroups
 
-// sign: midstate, R = PRF_msg, digest = H_msg, plan
 template <class P>
-__global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
+__host__ __device__ constexpr bool prep_has_long() { return P::hash == kSha2; }
+
+__device__ __forceinline__ bool prep_is_long(const BatchArgs& A, uint32_t i) {
+    return A.long_min && A.mlen[i] >= A.long_min;
+}
+
+template <class P>
+__device__ __forceinline__ void prep_plan_init(const BatchArgs& A, uint32_t row,
+                                               const uint32_t pub[P::nw], Plan& pl) {
+    if (A.key_mid) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
+    } else {
+        HashT<P>::template midstate<true>(pub, pl.mid);
+    }
+}
+
+// sign, one request: midstate, R = PRF_msg, digest = H_msg, plan; LONG: the
+// two-wave form over kw_s (prod: wave 1), else a lane pair
+template <class P, bool LONG>
+__device__ __forceinline__ void prep_sign_one(const BatchArgs& A, uint32_t i, bool store,
+                                              lds_u32* kw_s, bool prod) {
     using Hh = HashT<P>;
     constexpr int NW = P::nw;
-    // one serial hash chain per request over the whole message: ask the SIMD
-    // arbiter to favour these waves over concurrent bulk hypertree waves
-    __builtin_amdgcn_s_setprio(3);
-    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kPrepLanes;
-    if (i >= A.n_req) return;
-    const bool lead = (threadIdx.x & 1u) == 0;
     const uint32_t row = A.key_idx ? A.key_idx[i] : i;
     const uint8_t* sk = A.sk + (size_t)row * A.sk_stride;
     uint32_t skseed[NW], skprf[NW], pub[NW], pk[2 * NW], opt[NW], R[NW];
@@ -127,43 +157,68 @@ __global__ void __launch_bounds__(64) k_prep(BatchArgs A) {
     const uint8_t* m = A.msg + A.moff[i];
     const uint32_t ml = A.mlen[i];
     Plan pl;
-    if (A.key_mid) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
-    } else {
-        Hh::template midstate<true>(pub, pl.mid);
-    }
+    prep_plan_init<P>(A, row, pub, pl);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         pl.skseed[j] = j < NW ? skseed[j] : 0u;
         pl.pubseed[j] = j < NW ? pub[j] : 0u;
         pl.root[j] = 0;
     }
-    Hh::prf_msg(skprf, opt, m, ml, R);
     uint8_t dg[P::dgst_bytes];
-    Hh::template h_msg<P::dgst_bytes>(R, pk, m, ml, dg);
-    if (!lead) return;
+    if constexpr (LONG) {
+        Hh::template prf_msg<true>(skprf, opt, m, ml, R, kw_s, prod);
+        Hh::template h_msg<P::dgst_bytes, true>(R, pk, m, ml, dg, kw_s, prod);
+    } else {
+        (void)kw_s;
+        (void)prod;
+        Hh::prf_msg(skprf, opt, m, ml, R);
+        Hh::template h_msg<P::dgst_bytes>(R, pk, m, ml, dg);
+    }
+    if (!store) return;
     uint8_t* sig = A.sig + (size_t)i * A.sig_stride;
     store_val<P::hash, NW>(sig, R);
     plan_from_digest<P>(pl, dg);
     pl.status = 0;
     A.plan[i] = pl;
-    if (A.ht_count) {
-#pragma unroll 1
-        for (int layer = 0; layer < P::d; layer++) A.ht_count[(size_t)i * P::d + layer] = 0;
-    }
-    if (A.tile_ctr && i == 0) *A.tile_ctr = 0;
-    if (A.fors_count) A.fors_count[i] = 0;
 }
 
-// verify: midstate, digest = H_msg(R from the signature), plan
+// sign: midstate, R = PRF_msg, digest = H_msg, plan
 template <class P>
-__global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
+__global__ void __launch_bounds__(kPrepWg) k_prep(BatchArgs A) {
+    // one serial hash chain per request over the whole message: ask the SIMD
+    // arbiter to favour these waves over concurrent bulk hypertree waves
+    __builtin_amdgcn_s_setprio(3);
+    if (blockIdx.x >= A.prep_wgs) {
+        if constexpr (prep_has_long<P>()) {
+            __shared__ __align__(16) uint32_t kw_s[HashT<P>::kLongLdsWords];
+            const bool prod = threadIdx.x >= 64;
+#pragma unroll 1
+            for (uint32_t i = blockIdx.x - A.prep_wgs; i < A.n_req; i += gridDim.x - A.prep_wgs)
+                if (prep_is_long(A, i))
+                    prep_sign_one<P, true>(A, i, threadIdx.x == 0, (lds_u32*)kw_s, prod);
+        }
+        return;
+    }
+    const uint32_t i = (blockIdx.x * kPrepWg + threadIdx.x) / kPrepLanes;
+    if (i >= A.n_req) return;
+    if (A.ht_count) {
+#pragma unroll 1
+        for (int layer = 0; layer < P::d; layer++)
+            if (threadIdx.x % kPrepLanes == 0) A.ht_count[(size_t)i * P::d + layer] = 0;
+    }
+    if (A.tile_ctr && i == 0 && threadIdx.x == 0) *A.tile_ctr = 0;
+    if (A.fors_count && threadIdx.x % kPrepLanes == 0) A.fors_count[i] = 0;
+    if (prep_is_long(A, i)) return;
+    prep_sign_one<P, false>(A, i, (threadIdx.x & 1u) == 0, nullptr, false);
+}
+
+// verify, one request: midstate, digest = H_msg(R from the signature), plan
+// (LONG as for prep_sign_one)
+template <class P, bool LONG>
+__device__ __forceinline__ void prep_verify_one(const BatchArgs& A, uint32_t i, bool store,
+                                                lds_u32* kw_s, bool prod) {
     using Hh = HashT<P>;
     constexpr int NW = P::nw;
-    __builtin_amdgcn_s_setprio(3);  // serial message hash per request, as in k_prep
-    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kPrepLanes;
-    if (i >= A.n_req) return;
     const uint32_t row = A.key_idx ? A.key_idx[i] : i;
     const uint8_t* pkp = A.pk + (size_t)row * A.pk_stride;
     uint32_t pub[NW], pk[2 * NW], R[NW];
@@ -172,12 +227,7 @@ __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
     const uint8_t* sig = A.sig + (size_t)i * A.sig_stride;
     load_val<P::hash, NW>(sig, R);
     Plan pl;
-    if (A.key_mid) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) pl.mid[j] = A.key_mid[(size_t)row * 8 + j];
-    } else {
-        Hh::template midstate<true>(pub, pl.mid);
-    }
+    prep_plan_init<P>(A, row, pub, pl);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         pl.skseed[j] = 0;
@@ -185,11 +235,48 @@ __global__ void __launch_bounds__(64) k_vprep(BatchArgs A) {
         pl.root[j] = j < NW ? pk[NW + j] : 0u;
     }
     uint8_t dg[P::dgst_bytes];
-    Hh::template h_msg<P::dgst_bytes>(R, pk, A.msg + A.moff[i], A.mlen[i], dg);
-    if (threadIdx.x & 1u) return;  // the even lane of the pair stores
+    if constexpr (LONG) {
+        Hh::template h_msg<P::dgst_bytes, true>(R, pk, A.msg + A.moff[i], A.mlen[i], dg, kw_s,
+                                                prod);
+    } else {
+        (void)kw_s;
+        (void)prod;
+        Hh::template h_msg<P::dgst_bytes>(R, pk, A.msg + A.moff[i], A.mlen[i], dg);
+    }
+    if (!store) return;
     plan_from_digest<P>(pl, dg);
     pl.status = 0;
     A.plan[i] = pl;
+}
+
+// verify: midstate, digest = H_msg(R from the signature), plan
+template <class P>
+__global__ void __launch_bounds__(kPrepWg) k_vprep(BatchArgs A) {
+    __builtin_amdgcn_s_setprio(3);  // serial message hash per request, as in k_prep
+    if (blockIdx.x >= A.prep_wgs) {
+        if constexpr (prep_has_long<P>()) {
+            __shared__ __align__(16) uint32_t kw_s[HashT<P>::kLongLdsWords];
+            const bool prod = threadIdx.x >= 64;
+#pragma unroll 1
+            for (uint32_t i = blockIdx.x - A.prep_wgs; i < A.n_req; i += gridDim.x - A.prep_wgs)
+                if (prep_is_long(A, i))
+                    prep_verify_one<P, true>(A, i, threadIdx.x == 0, (lds_u32*)kw_s, prod);
+        }
+        return;
+    }
+    const uint32_t i = (blockIdx.x * kPrepWg + threadIdx.x) / kPrepLanes;
+    if (i >= A.n_req || prep_is_long(A, i)) return;
+    prep_verify_one<P, false>(A, i, (threadIdx.x & 1u) == 0, nullptr, false);
+}
+
+// grid of the prep kernels for n requests whose messages are at most
+// mlen_bound bytes (0: not known); sets A.prep_wgs / A.long_min
+template <class P>
+inline uint32_t prep_grid(BatchArgs& A, uint32_t n, uint32_t mlen_bound) {
+    A.prep_wgs = (uint32_t)(((uint64_t)n * kPrepLanes + kPrepWg - 1) / kPrepWg);
+    const bool lng = prep_has_long<P>() && (mlen_bound == 0 || mlen_bound >= kLongMsgBytes);
+    A.long_min = lng ? kLongMsgBytes : 0;
+    return A.prep_wgs + (lng ? (n < kLongWgs ? n : kLongWgs) : 0);
 }
 
 // resident-key midstates: mid[i] = state after pub_seed || 0^(64-n) for the

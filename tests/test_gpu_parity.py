@@ -331,3 +331,87 @@ def test_fors_pk_inside_and_outside_the_fors_tiles(name, S):
     assert np.array_equal(whole[256:], tail)
     for i in (0, 255, 256):
         assert bytes(whole[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), i
+
+
+def _device_sign_verify(pid, sk, pk, msgs, opts):
+    """Sign and verify through the device entry points, whose message lengths
+    the library does not see on the host (every prep path stays possible)."""
+    p = O.params(pid)
+    num = len(msgs)
+    moff = np.zeros(num, np.uint64)
+    buf = bytearray()
+    for i, m in enumerate(msgs):  # 4-byte aligned records
+        moff[i] = len(buf)
+        buf += m + bytes(-len(m) % 4)
+    D = C.DeviceBuffer
+    d_msg = D.from_array(np.frombuffer(bytes(buf) + bytes(4), np.uint8))
+    d_moff, d_mlen = D.from_array(moff), D.from_array(np.array([len(m) for m in msgs], np.uint32))
+    d_sk = D.from_array(np.ascontiguousarray(sk, np.uint8).reshape(-1))
+    d_pk = D.from_array(np.ascontiguousarray(pk, np.uint8).reshape(-1))
+    d_opt = D.from_array(np.frombuffer(b"".join(opts), np.uint8))
+    d_sig, d_st = D(num * p.sig_bytes), D(4 * num)
+    ws = D(max(C.workspace_bytes(pid, num, op) for op in (0, 1)))
+    st = C.Stream()
+    C.sign_device(pid, num, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr, d_opt.ptr,
+                  d_sig.ptr, p.sig_bytes, ws.ptr, st.handle)
+    d_st.fill(0x7F)
+    C.verify_device(pid, num, d_pk.ptr, p.pk_bytes, d_sig.ptr, p.sig_bytes, d_msg.ptr,
+                    d_moff.ptr, d_mlen.ptr, d_st.ptr, ws.ptr, st.handle)
+    st.sync()
+    return d_sig.download().reshape(num, p.sig_bytes), d_st.download(np.int32)
+
+
+@pytest.mark.parametrize("name", ["sha2-128f", "sha2-192f", "sha2-256f"])
+def test_sha2_long_messages_two_wave_prep(name):
+    """SHA-256 messages of 512 bytes or more are hashed by the prep kernels'
+    two-wave workgroups (wave 1 computes the schedules of the message-only
+    blocks, 32 per stage, while wave 0 runs the rounds).  Lengths around that
+    threshold, around 32 / 33 and 64 / 65 message-only blocks in either
+    message hash (n = 16, 24, 32 put 4 / 6 / 8 prefix words before the
+    message in PRF_msg and 12 / 18 / 24 in H_msg), with partial last words,
+    mixed with short messages in one batch: every signature equals the
+    oracle's on the host-buffer path (message lengths known to the library)
+    and the device path (not known), every one verifies, and a flipped
+    message byte fails exactly its item."""
+    pid = C.param_id(name)
+    p = O.params(pid)
+    lens = [511, 512, 513, 515, 0, 2064, 2096, 2127, 33, 2159, 2160, 4144, 4207, 4208,
+            1000, 8195, 65535]
+    seeds, msgs, opts = _inputs(pid, len(lens), 91 + pid, lens)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    print("%s: host-buffer sign of %d (lengths %s)" % (name, len(lens), lens), flush=True)
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    for i, n in enumerate(lens):
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), n
+    assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * len(lens)
+    bad = [bytearray(m) for m in msgs]
+    flip = [i for i, n in enumerate(lens) if n >= 512][::3]
+    for i in flip:
+        bad[i][lens[i] // 2] ^= 0x10
+    st = C.batch_verify(pid, pk, sig, [bytes(m) for m in bad])
+    assert [i for i in range(len(lens)) if st[i]] == flip
+    print("%s: device-path sign + verify" % name, flush=True)
+    dsig, dst = _device_sign_verify(pid, sk, pk, msgs, opts)
+    assert all(bytes(dsig[i]) == bytes(sig[i]) for i in range(len(lens)))
+    assert list(dst) == [0] * len(lens)
+
+
+def test_sha2_long_messages_beyond_the_long_workgroups():
+    """At most 512 two-wave workgroups run per launch, each taking the long
+    requests i, i + 512, ...: a 600-request batch whose long messages sit at
+    indices 3, 515 (the same workgroup as 3) and 599, among 1-byte ones."""
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    num = 600
+    lens = [1] * num
+    for i, n in ((3, 5000), (515, 512), (599, 3001)):
+        lens[i] = n
+    seeds, msgs, opts = _inputs(pid, num, 92, lens)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    print("sha2-128f: device-path sign + verify of %d" % num, flush=True)
+    dsig, dst = _device_sign_verify(pid, sk, pk, msgs, opts)
+    for i in (0, 3, 4, 514, 515, 598, 599):
+        assert bytes(dsig[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), i
+    assert list(dst) == [0] * num
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    assert np.array_equal(sig, dsig)
