@@ -132,9 +132,6 @@ __device__ __forceinline__ void sha256_compress_pair(uint32_t s[4], const uint32
 // a word pointer into LDS (address space 3), so that code reached through a
 // call still accesses it with ds_* instructions rather than flat ones
 typedef uint32_t __attribute__((address_space(3))) lds_u32;
-// a word pointer into global memory (address space 1): global_load rather
-// than flat, so message loads are not counted with LDS accesses
-typedef const uint32_t __attribute__((address_space(1))) gbl_u32;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef u32x4 __attribute__((address_space(3))) lds_u32x4;
 

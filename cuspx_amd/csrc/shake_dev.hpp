@@ -478,24 +478,43 @@ struct Shake {
 #pragma unroll
             for (int l = 0; l < 17; l++) s[l] ^= (qq >> 1) == l ? c : 0u;
         };
+        // word by word up to the first block boundary, then the blocks made
+        // only of message words (block b + 1 loads while block b permutes),
+        // then the words left over
+        auto words = [&](uint32_t stop) {
 #pragma unroll 1
-        while (i < full) {
-            if (q == 0 && i + 34 <= full) {
+            while (i < stop) {
+                put(q, mw[i]);
+                q++;
+                i++;
+                if (q == 34) {
+                    keccak_f_il_pair(s);
+                    q = 0;
+                }
+            }
+        };
+        const uint32_t head = q == 0 ? 0u : 34u - (uint32_t)q;
+        words(head < full ? head : full);
+        const uint32_t nfb = q == 0 ? (full - i) / 34 : 0u;
+        if (nfb) {
+            const gbl_u32* g = (gbl_u32*)mw + i;  // address-space cast
+            uint32_t cur[34];
+#pragma unroll
+            for (int l = 0; l < 34; l++) cur[l] = g[l];
+#pragma unroll 1
+            for (uint32_t b = 0; b < nfb; b++) {
 #pragma unroll
                 for (int l = 0; l < 17; l++)
-                    s[l] ^= il_even16(mw[i + 2 * l] >> h) | (il_even16(mw[i + 2 * l + 1] >> h) << 16);
+                    s[l] ^= il_even16(cur[2 * l] >> h) | (il_even16(cur[2 * l + 1] >> h) << 16);
+                if (b + 1 < nfb) {
+#pragma unroll
+                    for (int l = 0; l < 34; l++) cur[l] = g[34 * (b + 1) + l];
+                }
                 keccak_f_il_pair(s);
-                i += 34;
-                continue;
             }
-            put(q, mw[i]);
-            q++;
-            i++;
-            if (q == 34) {
-                keccak_f_il_pair(s);
-                q = 0;
-            }
+            i += 34 * nfb;
         }
+        words(full);
         uint32_t last = rem ? (mw[full] & (0xFFFFFFFFu >> (8 * (4 - rem)))) : 0u;
         last |= 0x1Fu << (8 * rem);
         put(q, last);

@@ -62,6 +62,10 @@ static_assert(sizeof(Plan) <= 512, "plan record too large");
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
 
+// a word pointer into global memory (address space 1): global_load rather
+// than flat, so message loads are not counted with LDS accesses
+typedef const uint32_t __attribute__((address_space(1))) gbl_u32;
+
 __device__ __forceinline__ uint32_t rotr32(uint32_t x, int c) {
     return __builtin_amdgcn_alignbit(x, x, c);
 }
