@@ -443,21 +443,20 @@ struct Sha2 {
     // SHA-256 continuing from st over (pre[0..NPRE) || m[0..mlen)), where
     // `before` bytes were already absorbed.  m is 4-byte aligned; bytes past
     // mlen are never read beyond the word holding the last byte.  Blocks made
-    // only of whole message words are loaded directly; the head (prefix) and
-    // the tail (last partial word, padding, length) go word by word.
+    // only of whole message words are loaded as blocks (stream_full, or
+    // stream_msg_long's producing wave); the head (prefix) and the tail (last
+    // partial word, padding, length) go word by word.
     //
-    // stream_range runs the words [kk, kk_end) of that stream (kk a multiple
-    // of 16; kk_end clipped to the padded end), stream_msg all of them.
+    // stream_range runs the words [kk, kk_end) of that stream word by word
+    // (kk a multiple of 16; kk_end clipped to the padded end).
     template <int NPRE>
     __device__ static void stream_range(uint32_t st[8], const uint32_t pre[NPRE],
                                         const uint8_t* m, uint32_t mlen, uint32_t before,
                                         uint32_t kk, uint32_t kk_end) {
         const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
         const uint32_t ndata = NPRE + (mlen + 4) / 4;  // words holding data or 0x80
-        const uint32_t nfull = NPRE + mlen / 4;        // words made of message bytes only
         const uint32_t total = ((ndata + 2 + 15) / 16) * 16;
         if (kk_end > total) kk_end = total;
-        (void)nfull;
         const gbl_u32* mw = (gbl_u32*)m;  // address-space cast
         uint32_t sl[8], b[16];  // the chaining value stays in registers
 #pragma unroll
