@@ -132,12 +132,14 @@ struct FamilyImpl {
         T.ctr = n0 ? A0.tile_ctr : n1 ? A1.tile_ctr : A2.tile_ctr;
         (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, "family");
-        if (n0) hipLaunchKernelGGL((k_prep<P0>), dim3(prep_grid<P0>(A0, n0, b[0].mlen_bound)),
-                                   dim3(kPrepWg), 0, s, A0);
-        if (n1) hipLaunchKernelGGL((k_prep<P1>), dim3(prep_grid<P1>(A1, n1, b[1].mlen_bound)),
-                                   dim3(kPrepWg), 0, s, A1);
-        if (n2) hipLaunchKernelGGL((k_prep<P2>), dim3(prep_grid<P2>(A2, n2, b[2].mlen_bound)),
-                                   dim3(kPrepWg), 0, s, A2);
+        // prep_grid sets fields of the BatchArgs: call it before the launch
+        // that copies them
+        const uint32_t g0 = n0 ? prep_grid<P0>(A0, n0, b[0].mlen_bound) : 0;
+        const uint32_t g1 = n1 ? prep_grid<P1>(A1, n1, b[1].mlen_bound) : 0;
+        const uint32_t g2 = n2 ? prep_grid<P2>(A2, n2, b[2].mlen_bound) : 0;
+        if (n0) hipLaunchKernelGGL((k_prep<P0>), dim3(g0), dim3(kPrepWg), 0, s, A0);
+        if (n1) hipLaunchKernelGGL((k_prep<P1>), dim3(g1), dim3(kPrepWg), 0, s, A1);
+        if (n2) hipLaunchKernelGGL((k_prep<P2>), dim3(g2), dim3(kPrepWg), 0, s, A2);
         prof_mark(s, "k_prep");
         static const uint32_t resident = [] {
             int per_cu = 0, cus = 0, dev = 0;

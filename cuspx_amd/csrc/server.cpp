@@ -179,8 +179,9 @@ int Server::launch(Slot& sl, int pid, int op, std::vector<ReqP>& reqs) {
     b.msg = sl.d;
     b.moff = (const uint64_t*)(sl.d + o_moff);
     b.mlen = (const uint32_t*)(sl.d + o_mlen);
+    b.mlen_bound = 1;  // the longest message (0 would mean "not known")
     for (uint32_t i = 0; i < num; i++)
-        b.mlen_bound = std::max<uint32_t>(b.mlen_bound, (uint32_t)reqs[i]->msg.size() + 1);
+        b.mlen_bound = std::max<uint32_t>(b.mlen_bound, (uint32_t)reqs[i]->msg.size());
     b.key_idx = (const uint32_t*)(sl.d + o_row);
     b.key_mid = kt.d_mid;
     b.ws = sl.d + o_ws;

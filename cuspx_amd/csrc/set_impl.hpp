@@ -244,8 +244,8 @@ struct SetImpl {
         constexpr uint32_t fors_tpw = 256u >> (P::a - J);
         (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, set_name_);
-        hipLaunchKernelGGL((k_prep<P>), dim3(prep_grid<P>(A, n, b.mlen_bound)), dim3(kPrepWg), 0,
-                           s, A);
+        const uint32_t prep_grid_wgs = prep_grid<P>(A, n, b.mlen_bound);  // sets fields of A
+        hipLaunchKernelGGL((k_prep<P>), dim3(prep_grid_wgs), dim3(kPrepWg), 0, s, A);
         prof_mark(s, "k_prep");
         // one persistent launch for FORS + hypertree tiles: FORS then overlaps
         // the hypertree instead of running before it
@@ -292,8 +292,8 @@ struct SetImpl {
         const int G = Lay::vsplit(pid_, n);
         (void)hipGetLastError();  // report only errors of these launches
         prof_begin(s, set_name_);
-        hipLaunchKernelGGL((k_vprep<P>), dim3(prep_grid<P>(A, n, b.mlen_bound)), dim3(kPrepWg),
-                           0, s, A);
+        const uint32_t prep_grid_wgs = prep_grid<P>(A, n, b.mlen_bound);  // sets fields of A
+        hipLaunchKernelGGL((k_vprep<P>), dim3(prep_grid_wgs), dim3(kPrepWg), 0, s, A);
         prof_mark(s, "k_vprep");
         hipLaunchKernelGGL((k_vfors<P>), dim3(cdiv((uint64_t)n * P::k, 64)), dim3(64), 0, s, A);
         prof_mark(s, "k_vfors");
