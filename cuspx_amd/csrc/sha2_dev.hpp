@@ -70,6 +70,46 @@ __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) 
     st[7] += h;
 }
 
+// Rounds [R0, R1) of a compression from the working variables v (a..h after
+// round R0 - 1, or the chaining value for R0 = 0); w holds the whole block,
+// whose rolling schedule needs W[0..16) whatever R0 is.
+template <int R0, int R1>
+__device__ __forceinline__ void sha256_rounds(uint32_t v[8], uint32_t w[16]) {
+    uint32_t a = v[0], b = v[1], c = v[2], d = v[3], e = v[4], f = v[5], g = v[6], h = v[7];
+#pragma unroll
+    for (int i = R0; i < R1; i++) {
+        uint32_t wi;
+        if (i < 16) {
+            wi = w[i];
+        } else {
+            uint32_t x = w[(i - 15) & 15], y = w[(i - 2) & 15];
+            uint32_t s0 = xor3(rotr32(x, 7), rotr32(x, 18), x >> 3);
+            uint32_t s1 = xor3(rotr32(y, 17), rotr32(y, 19), y >> 10);
+            wi = s1 + w[(i - 7) & 15] + s0 + w[i & 15];
+            w[i & 15] = wi;
+        }
+        uint32_t t1 = h + xor3(rotr32(e, 6), rotr32(e, 11), rotr32(e, 25)) + bit_ch(e, f, g) +
+                      kSha256K[i] + wi;
+        uint32_t t2 = xor3(rotr32(a, 2), rotr32(a, 13), rotr32(a, 22)) + bit_maj(a, b, c);
+        h = g;
+        g = f;
+        f = e;
+        e = d + t1;
+        d = c;
+        c = b;
+        b = a;
+        a = t1 + t2;
+    }
+    v[0] = a;
+    v[1] = b;
+    v[2] = c;
+    v[3] = d;
+    v[4] = e;
+    v[5] = f;
+    v[6] = g;
+    v[7] = h;
+}
+
 // ------------------------------------------------------------ lane pairs
 //
 // One compression on a lane pair (lanes 2i and 2i+1 of a wave, both active,
@@ -340,6 +380,32 @@ struct Sha2 {
     __device__ static __forceinline__ void F(const Key& k, const Adr& a, const uint32_t in[NW],
                                              uint32_t out[NW]) {
         thash_words<NW>(k, a, in, out);
+    }
+
+    // F along one chain: its block's first five words (the address up to the
+    // chain index; the hash index sits in word 5) do not change from step to
+    // step, so rounds 0-4 run once per chain (F_prefix) and each step runs
+    // rounds 5-63 from there (F_step).  F is one block for every n.
+    __device__ static __forceinline__ void F_prefix(const Key& k, const Adr& a, uint32_t pre[8]) {
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = i < 5 ? a.w[i] : 0u;
+#pragma unroll
+        for (int i = 0; i < 8; i++) pre[i] = k.mid[i];
+        sha256_rounds<0, 5>(pre, w);
+    }
+    __device__ static __forceinline__ void F_step(const Key& k, const uint32_t pre[8],
+                                                  const Adr& a, const uint32_t in[NW],
+                                                  uint32_t out[NW]) {
+        static_assert((6 + NW + 2 + 15) / 16 == 1, "F is one block");
+        uint32_t w[16], v[8];
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = block_word<NW>(a, in, 0, i);
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = pre[i];
+        sha256_rounds<5, 64>(v, w);
+#pragma unroll
+        for (int i = 0; i < NW; i++) out[i] = k.mid[i] + v[i];
     }
 
     // H: two input values (Merkle node)

@@ -1454,6 +1454,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
         Adr wa;
         uint32_t v[NW];
         typename std::conditional<SHA2, int, typename Shake<P::n>::ChainIl>::type ch{};
+        uint32_t pre[SHA2 ? 8 : 1];  // SHA-256: the chain's rounds 0-4 (Sha2::F_prefix)
 #pragma unroll
         for (int w = 0; w < NW; w++) v[w] = 0;
 #pragma unroll
@@ -1473,6 +1474,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
             if constexpr (SHA2) {
 #pragma unroll
                 for (int i = 0; i < 8; i++) key.mid[i] = key_s[r][i];
+                Hh::F_prefix(key, wa, pre);
             } else {
 #pragma unroll
                 for (int i = 0; i < NW; i++) key.pub[i] = key_s[r][i];
@@ -1495,7 +1497,7 @@ __global__ void __launch_bounds__(256) k_vht(BatchArgs A) {
             }
             if constexpr (SHA2) {
                 adr_hash(wa, s);
-                Hh::F(key, wa, v, v);
+                Hh::F_step(key, pre, wa, v, v);
             } else {
                 Hh::F_il(ch, s, v, v);
             }
