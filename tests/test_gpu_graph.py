@@ -76,3 +76,41 @@ def test_mixed_set_graph_equals_per_set_batches():
             want = C.batch_sign(pid, np.frombuffer(sk, np.uint8), [m],
                                 np.frombuffer(opt, np.uint8))
             assert s == bytes(want[0])
+
+
+def test_graph_replays_with_message_lengths_changing():
+    """A captured batch does not know its message lengths (they sit in device
+    memory), so its prep launch keeps the two-wave workgroups of long SHA-256
+    messages: replays whose messages move across the 512-byte threshold and
+    up to 6000 bytes equal the eager batch every time."""
+    pid = C.param_id("sha2-128f")
+    p = C.params(pid)
+    num, cap = 8, 6000
+    pk, sk, _, opts = _dev_batch(pid, num, 31)
+    D = C.DeviceBuffer
+    d_sk, d_opt = D.from_array(sk), D.from_array(opts)
+    d_msg = D(num * cap + 16)
+    d_moff = D.from_array(np.arange(num, dtype=np.uint64) * cap)
+    d_mlen = D.from_array(np.full(num, 32, np.uint32))
+    d_sig = D(num * p.sig_bytes)
+    ws = D(C.workspace_bytes(pid, num, C.OP_SIGN))
+    g = C.graph_sign(pid, num, d_sk.ptr, p.sk_bytes, d_msg.ptr, d_moff.ptr, d_mlen.ptr,
+                     d_opt.ptr, d_sig.ptr, p.sig_bytes, ws.ptr)
+    st = C.Stream()
+    rng = random.Random(5)
+    for rep, lens in enumerate([[32] * num, [511, 512, 600, 5000, 1, 2047, 2048, 6000],
+                                [6000] * num, [0, 3, 40, 100, 513, 4208, 5999, 64]]):
+        msgs = [rng.randbytes(n) for n in lens]
+        buf = bytearray(num * cap + 16)
+        for i, m in enumerate(msgs):
+            buf[i * cap:i * cap + len(m)] = m
+        d_msg.upload(np.frombuffer(bytes(buf), np.uint8))
+        d_mlen.upload(np.array(lens, np.uint32))
+        C.batch_sign(pid, sk[:1], [b"eager %d" % rep], opts[:p.n])
+        print("replay %d, lengths %s" % (rep, lens), flush=True)
+        g.launch(st)
+        st.sync()
+        got = d_sig.download().reshape(num, p.sig_bytes)
+        want = C.batch_sign(pid, sk, msgs, opts)
+        assert (got == want).all(), rep
+    g.close()

@@ -415,3 +415,22 @@ def test_sha2_long_messages_beyond_the_long_workgroups():
     assert list(dst) == [0] * num
     sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
     assert np.array_equal(sig, dsig)
+
+
+def test_sha2_small_set_long_messages():
+    """The two-wave prep path on a small set (SHA2-128s: its own digest split
+    into FORS indices and a 54-bit tree index): messages at the 512-byte
+    threshold, at a stage boundary and at 64 KiB, on the host-buffer and the
+    device path, equal to the oracle's and verified."""
+    pid = C.param_id("sha2-128s")
+    lens = [511, 512, 4208, 65535]
+    seeds, msgs, opts = _inputs(pid, len(lens), 93, lens)
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    print("sha2-128s: host-buffer sign of %d (lengths %s)" % (len(lens), lens), flush=True)
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    for i, n in enumerate(lens):
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, msgs[i], bytes(sk[i]), opts[i]), n
+    assert list(C.batch_verify(pid, pk, sig, msgs)) == [0] * len(lens)
+    dsig, dst = _device_sign_verify(pid, sk, pk, msgs, opts)
+    assert np.array_equal(dsig, sig)
+    assert list(dst) == [0] * len(lens)
