@@ -1,5 +1,5 @@
 // set_impl.hpp -- host-side launch sequences for one parameter set.
-// Each set_<name>.hip instantiates SetImpl<P, J> and registers its SetOps.
+// Each set_<name>.hip instantiates SetImpl<P, J, JB> and registers its SetOps.
 #pragma once
 #include "engine.hpp"
 #include "profile.hpp"
@@ -160,9 +160,15 @@ struct Layout {
     }
 };
 
-template <class P, int J>
+// J: FORS leaves per lane (2^(a-J) lanes per tree) in the tiles of split
+// S > 1 (small batches: short tiles keep FORS off the critical path) and in
+// the unfused k_fors; JB: the same in the fused launch at S = 1 (large
+// batches, where longer tiles spend less on tile set-up and barriers)
+template <class P, int J, int JB>
 struct SetImpl {
     using Lay = Layout<P>;
+    template <int S>
+    static constexpr int fors_j() { return S == 1 ? JB : J; }
 
     static int pid_;
     static const char* set_name_;
@@ -180,10 +186,11 @@ struct SetImpl {
     template <int S>
     static bool launch_fused(const BatchArgs& A, uint32_t n, hipStream_t s) {
         if constexpr (HtShape<P, S>::WG == 256) {
+            constexpr int FJ = fors_j<S>();
             static const uint32_t resident = [] {
                 int per_cu = 0, cus = 0, dev = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &per_cu, reinterpret_cast<const void*>(&k_sign_fused<P, J, S>), 256, 0) !=
+                        &per_cu, reinterpret_cast<const void*>(&k_sign_fused<P, FJ, S>), 256, 0) !=
                         hipSuccess ||
                     hipGetDevice(&dev) != hipSuccess ||
                     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
@@ -191,11 +198,11 @@ struct SetImpl {
                     return 256u;
                 return (uint32_t)((per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1));
             }();
-            constexpr uint32_t fors_tpw = 256u >> (P::a - J);
+            constexpr uint32_t fors_tpw = 256u >> (P::a - FJ);
             const uint32_t n_ht = Lay::template ht_blocks<S>(n * P::d);
             const uint32_t n_fors = cdiv((uint64_t)n * P::k, fors_tpw);
             const uint32_t grid = n_ht + n_fors < resident ? n_ht + n_fors : resident;
-            hipLaunchKernelGGL((k_sign_fused<P, J, S>), dim3(grid), dim3(256), 0, s, A, n_ht, n_fors);
+            hipLaunchKernelGGL((k_sign_fused<P, FJ, S>), dim3(grid), dim3(256), 0, s, A, n_ht, n_fors);
             return true;
         }
         return false;
@@ -371,15 +378,15 @@ struct SetImpl {
     }
 };
 
-template <class P, int J>
-int SetImpl<P, J>::pid_ = -1;
-template <class P, int J>
-const char* SetImpl<P, J>::set_name_ = "";
+template <class P, int J, int JB>
+int SetImpl<P, J, JB>::pid_ = -1;
+template <class P, int J, int JB>
+const char* SetImpl<P, J, JB>::set_name_ = "";
 
 }  // namespace spx
 
-#define SPX_REGISTER_SET(SYM, NAME, PID, PARAMS, J)                        \
+#define SPX_REGISTER_SET(SYM, NAME, PID, PARAMS, J, JB)                    \
     extern "C" const spx::SetOps* SYM(void) {                              \
-        static const spx::SetOps o = spx::SetImpl<PARAMS, J>::ops(NAME, PID); \
+        static const spx::SetOps o = spx::SetImpl<PARAMS, J, JB>::ops(NAME, PID); \
         return &o;                                                         \
     }

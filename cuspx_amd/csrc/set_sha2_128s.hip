@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kSha2, 16, 63, 7, 12, 14>;
-SPX_REGISTER_SET(cuspx_set_sha2_128s, "sha2-128s", 0, SetParams, 4)
+SPX_REGISTER_SET(cuspx_set_sha2_128s, "sha2-128s", 0, SetParams, 4, 4)

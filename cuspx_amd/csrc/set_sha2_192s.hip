@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kSha2, 24, 63, 7, 14, 17>;
-SPX_REGISTER_SET(cuspx_set_sha2_192s, "sha2-192s", 2, SetParams, 6)
+SPX_REGISTER_SET(cuspx_set_sha2_192s, "sha2-192s", 2, SetParams, 6, 6)

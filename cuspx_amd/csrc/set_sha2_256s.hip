@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kSha2, 32, 64, 8, 14, 22>;
-SPX_REGISTER_SET(cuspx_set_sha2_256s, "sha2-256s", 4, SetParams, 6)
+SPX_REGISTER_SET(cuspx_set_sha2_256s, "sha2-256s", 4, SetParams, 6, 6)

@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kShake, 16, 66, 22, 6, 33>;
-SPX_REGISTER_SET(cuspx_set_shake256_128f, "shake256-128f", 7, SetParams, 2)
+SPX_REGISTER_SET(cuspx_set_shake256_128f, "shake256-128f", 7, SetParams, 2, 2)

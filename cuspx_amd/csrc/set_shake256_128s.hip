@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kShake, 16, 63, 7, 12, 14>;
-SPX_REGISTER_SET(cuspx_set_shake256_128s, "shake256-128s", 6, SetParams, 4)
+SPX_REGISTER_SET(cuspx_set_shake256_128s, "shake256-128s", 6, SetParams, 4, 4)

@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kShake, 24, 63, 7, 14, 17>;
-SPX_REGISTER_SET(cuspx_set_shake256_192s, "shake256-192s", 8, SetParams, 6)
+SPX_REGISTER_SET(cuspx_set_shake256_192s, "shake256-192s", 8, SetParams, 6, 6)

@@ -2,4 +2,4 @@
 #include "set_impl.hpp"
 
 using SetParams = spx::Params<spx::kShake, 32, 64, 8, 14, 22>;
-SPX_REGISTER_SET(cuspx_set_shake256_256s, "shake256-256s", 10, SetParams, 6)
+SPX_REGISTER_SET(cuspx_set_shake256_256s, "shake256-256s", 10, SetParams, 6, 6)

@@ -193,8 +193,9 @@ def test_batch_determinism_and_order_independence():
     assert (s3 == s1[perm]).all()
 
 
-@pytest.mark.parametrize("name", ["sha2-128f", "sha2-128s", "shake256-128f", "sha2-256f",
-                                  "shake256-192f", "sha2-256s", "shake256-128s"])
+@pytest.mark.parametrize("name", ["sha2-128f", "sha2-128s", "shake256-128f", "sha2-192f",
+                                  "sha2-256f", "shake256-192f", "shake256-256f", "sha2-256s",
+                                  "shake256-128s"])
 def test_every_leaf_split_matches_oracle(name):
     pid = C.param_id(name)
     p = O.params(pid)
@@ -280,12 +281,14 @@ def test_verify_ragged_workgroups(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,num,S", [("sha2-128f", 1600, 1), ("shake256-128f", 1600, 1),
+                                        ("sha2-192f", 300, 1), ("sha2-256f", 300, 1),
                                         ("sha2-128s", 300, 2)])
 def test_large_batch_launch_paths(name, num, S):
     """Batches big enough for the full-chip launch shapes: the fused persistent
-    signing kernel (SHA-256), the separate FORS and hypertree kernels (SHAKE
-    at a full chip, 128s with 512/1024-lane subtrees).  A sample is compared
-    with the oracle byte for byte; every signature must verify."""
+    signing kernel (SHA-256; at S = 1 its FORS tiles hold 32-64 trees, so a
+    tile spans request boundaries), the separate FORS and hypertree kernels
+    (SHAKE at a full chip, 128s with 512/1024-lane subtrees).  A sample is
+    compared with the oracle byte for byte; every signature must verify."""
     pid = C.param_id(name)
     p = O.params(pid)
     rng = random.Random(4242 + pid)
