@@ -30,8 +30,10 @@ import numpy as np
 import cuspx_amd as C
 
 OPS = {"sign": 0, "keygen": 2, "verify": 1}
-BATCHES_FAST = (1, 4, 16, 64, 256, 1024, 4096)
-BATCHES_SMALL = (1, 4, 16, 64, 256, 1024)
+# powers of two: a winner holds from its batch up to the next grid point, so a
+# 4x grid left e.g. 2048 requests on the split timed best at 1024
+BATCHES_FAST = tuple(1 << i for i in range(13))   # 1 .. 4096
+BATCHES_SMALL = tuple(1 << i for i in range(11))  # 1 .. 1024
 
 
 def _time(fn, stream, reps):
