@@ -433,7 +433,8 @@ static int sign_enqueue(const SetOps* o, Lane& L, size_t budget, uint32_t num,
     p += al16((uint64_t)num * o->sk_bytes);
     uint8_t* pm = p;
     memset(pm, 0, mp.bytes);
-    for (uint32_t i = 0; i < num; i++) memcpy(pm + mp.off[i], msg + moff[i], mlen[i]);
+    for (uint32_t i = 0; i < num; i++)  // msg may be null when every message is empty
+        if (mlen[i]) memcpy(pm + mp.off[i], msg + moff[i], mlen[i]);
     p += mp.bytes;
     memcpy(p, mp.off.data(), 8ull * num);
     p += al16(8ull * num);
@@ -532,7 +533,8 @@ static int verify_enqueue(const SetOps* o, Lane& L, size_t budget,
     const uint64_t sig_region = al16((uint64_t)num * sb);
     p += sig_region;
     memset(p, 0, mp.bytes);
-    for (uint32_t i = 0; i < num; i++) memcpy(p + mp.off[i], msg + moff[i], mlen[i]);
+    for (uint32_t i = 0; i < num; i++)
+        if (mlen[i]) memcpy(p + mp.off[i], msg + moff[i], mlen[i]);
     p += mp.bytes;
     memcpy(p, mp.off.data(), 8ull * num);
     p += al16(8ull * num);

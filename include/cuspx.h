@@ -255,7 +255,8 @@ int spx_server_add_key(spx_server *s, int pid, const uint8_t *sk, const uint8_t 
 /* New requests on the key are refused (-4) from now on; requests already
  * queued or running on it complete normally, and its row is reused by a
  * later spx_server_add_key once they have (under a new key_id).  -4 for an
- * unknown or already removed key. */
+ * unknown or already removed key.  The generation is 11 bits: a key_id kept
+ * past 2048 reuses of its row may name that row's current key again. */
 int spx_server_remove_key(spx_server *s, int32_t key_id);
 /* op 0 = sign (aux = optrand[n] or NULL to draw it like crypto_sign_signature),
  * op 1 = verify (aux = signature).  -5 if the request can never fit one

@@ -180,6 +180,31 @@ def test_face_mdp_batch_surface():
     assert bytes(mo[1]) == bytes(m[32:64])
 
 
+def test_empty_messages_with_a_null_message_pointer():
+    """A batch whose messages are all empty may pass msg = NULL (nothing is
+    read from it): the signatures equal the oracle's for the empty message,
+    and they verify with msg = NULL as well."""
+    import ctypes
+    pid = C.param_id("sha2-128f")
+    p = O.params(pid)
+    num = 3
+    seeds, _, opts = _inputs(pid, num, 777, [0])
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    opt = np.frombuffer(b"".join(opts), np.uint8).reshape(num, p.n)
+    moff = np.zeros(num, np.uint64)
+    mlen = np.zeros(num, np.uint32)
+    sig = np.zeros((num, p.sig_bytes), np.uint8)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert C.lib().spx_batch_sign(pid, num, ptr(sk), None, ptr(moff), ptr(mlen), ptr(opt),
+                                  ptr(sig)) == 0
+    for i in range(num):
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, b"", bytes(sk[i]), opts[i]), i
+    st = np.full(num, -7, np.int32)
+    assert C.lib().spx_batch_verify(pid, num, ptr(pk), ptr(sig), None, ptr(moff), ptr(mlen),
+                                    ptr(st)) == 0
+    assert list(st) == [0] * num
+
+
 def test_batch_determinism_and_order_independence():
     pid = C.param_id("sha2-128f")
     seeds, msgs, opts = _inputs(pid, 8, 9, [7, 32, 100])
