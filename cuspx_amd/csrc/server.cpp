@@ -7,11 +7,12 @@
 //   queue per (set, op, message-size class: <= 2 KiB, <= 16 KiB, larger) so
 //   short messages never wait for the serial hash of a long one in the same
 //   launch; batches of the longest class may be held to `long_slots` slots
-//   at once, so their long serial hashes leave slots to short requests.  A scheduler thread keeps up to `slots` batches in
-//   flight, each on its own stream with its own pinned staging and device
-//   arena.  When a slot frees, the oldest queue is packed into the next
-//   batch: immediately if the GPU is idle or the batch is full, otherwise
-//   after at most `batch_window_us` -- so requests that arrive while other
+//   at once, so their long serial hashes leave slots to short requests.  A
+//   scheduler thread keeps up to `slots` batches in flight, each on its own
+//   stream with its own pinned staging and device arena.  When a slot frees,
+//   the oldest queue is packed into the next batch: immediately while fewer
+//   than half the slots are busy or when the batch is full, otherwise after
+//   at most `batch_window_us` -- so requests that arrive while other
 //   batches run are admitted as slots free up and overlap on the device.
 // * Admission: the device-memory budget is split over the slots; a batch
 //   takes requests while its inputs, outputs and workspace fit one slot's
@@ -303,9 +304,15 @@ void Server::loop() {
                 hbytes += hneed;
                 take++;
             }
+            // launch now while at least half the slots are idle (a wait
+            // there only adds latency: requests spread over up to 18
+            // queues rarely meet in one window at moderate rates); once
+            // most slots are busy, hold a batch until it is full or its
+            // oldest request has waited the batch window
             const auto age = Clock::now() - best->front()->t_submit;
             const bool full = take >= cfg.max_batch || take < best->size();
-            if (!(inflight == 0 || full || age >= window)) {
+            const bool idle = 2 * inflight < (int)slots.size();
+            if (!(idle || full || age >= window)) {
                 wake = std::min(wake, best->front()->t_submit + window);
                 continue;
             }

@@ -229,7 +229,8 @@ typedef struct {
     uint32_t max_batch;         /* requests per launch (0: 4096) */
     uint32_t slots;             /* batches in flight (0: 4) */
     uint32_t max_msg_bytes;     /* largest message accepted (0: 65536) */
-    uint32_t batch_window_us;   /* longest wait for a partial batch while the GPU is busy */
+    uint32_t batch_window_us;   /* longest wait for a partial batch while at least half
+                                   the slots are busy (fewer: it launches at once) */
     uint32_t max_keys_per_set;  /* resident keys per parameter set (0: 1024, at most 65536) */
     uint32_t long_slots;        /* most slots running batches of messages over 16 KiB
                                    at once (0: all slots); fewer trades p99 and peak
