@@ -277,6 +277,34 @@ def test_every_verify_width(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sha2-128f", "sha2-128s", "sha2-256f", "shake256-128f",
+                                  "shake256-192s"])
+def test_every_signature_hash_is_checked(name):
+    """One bit flipped in each n-byte hash of a signature (R, every FORS
+    secret and auth node, every WOTS chain value, every hypertree auth node)
+    must fail verification: the test the reference keeps behind the never
+    defined SPX_TEST_INVALIDSIG (test/spx.cu:168-184), here as one batch of
+    sig_bytes / n tampered copies plus the original, which must verify.  A
+    sample of the copies is also rejected by the oracle."""
+    pid = C.param_id(name)
+    p = O.params(pid)
+    seeds, msgs, opts = _inputs(pid, 1, 4321 + pid, [57])
+    pk, sk = C.batch_keygen(pid, np.frombuffer(seeds[0], np.uint8))
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(opts[0], np.uint8))[0]
+    segs = p.sig_bytes // p.n
+    bad = np.tile(sig, (segs + 1, 1))
+    for k in range(segs):  # segment k: byte k % n of it, bit k % 8
+        bad[k, k * p.n + k % p.n] ^= 1 << (k % 8)
+    st = C.batch_verify(pid, np.tile(pk[0], (segs + 1, 1)), bad, msgs * (segs + 1))
+    assert st[segs] == 0
+    passed = [k for k in range(segs) if st[k] == 0]
+    assert passed == [], "%s: %d of %d tampered hashes verified, first %s" % (
+        name, len(passed), segs, passed[:8])
+    for k in range(0, segs, max(1, segs // 12)):
+        assert O.verify(pid, bytes(bad[k]), msgs[0], bytes(pk[0])) != 0, k
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["sha2-128f", "shake256-256f"])
 def test_verify_ragged_workgroups(name):
     """37 requests: at G = 16 two full workgroups and one with 5 of 16 request
