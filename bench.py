@@ -104,7 +104,7 @@ def host_cpus():
 
 def cpu_baseline(pid, seconds, max_procs):
     """The oracle (single-threaded C restatement of the reference host path)
-    timed on the host cores, as BASELINE.md plans it:
+    timed on the host cores:
 
     * one core: one process pinned to one physical core, 2 warm-up then 10
       timed signatures (32-byte messages) and 10 verifications; medians on
@@ -496,8 +496,9 @@ def main():
                          "5 s each, until sign p99 passes 50 ms")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X "
-              "(signatures shard by request; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: signatures shard by request with no exchange "
+              "between GPUs, so one MI355X's throughput is the figure. Use --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
