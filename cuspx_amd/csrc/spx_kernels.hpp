@@ -1167,8 +1167,10 @@ constexpr int max_int(int a, int b) { return a > b ? a : b; }
 // passes the last tile.
 // SHA-256: 4 waves per SIMD, as k_ht (5, at 96 VGPRs, keeps the F loop free
 // of spills but spills in T_len, and took 13.87-13.90 ms at SHA2-128f / 4096
-// against 13.73-13.81); SHAKE256 (50-word sponge): 3, as k_ht -- forcing 4
-// there spills the permutation state
+// against 13.73-13.81, both with 16 lanes per FORS tree); SHAKE256 (50-word
+// sponge): 3, as k_ht -- at 4 (128 VGPRs) every fast set signed slower, also
+// with the chain-step loop kept free of spills (SHAKE256-128f at 4096 40.4
+// against 39.95 ms)
 template <class P, int J, int S>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P::hash == kSha2 ? 4 : 3)))
 k_sign_fused(BatchArgs A, uint32_t n_ht, uint32_t n_fors) {
