@@ -1,4 +1,6 @@
-"""Builds libcuspx.so in-tree with hipcc for gfx950.
+"""Builds libcuspx.so in-tree with hipcc for gfx950, then the twelve per-set
+libraries libspx_<set>.so (csrc/perset.c with gcc: the api.h names bound to
+one parameter set, as the reference builds one library per set).
 
     python -m cuspx_amd.build            # incremental
     python -m cuspx_amd.build --force    # rebuild everything
@@ -67,7 +69,40 @@ def build(force=False, jobs=None, verbose=False):
         if p.returncode != 0:
             raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
         os.replace(tmp, LIB)
+    build_perset(force)
     return LIB
+
+
+def perset_names():
+    """The parameter sets, from the per-set units: set_sha2_128f.hip -> sha2-128f."""
+    names = []
+    for src in sorted(glob.glob(os.path.join(CSRC, "set_*.hip"))):
+        stem = os.path.basename(src)[len("set_"):-len(".hip")]
+        names.append(stem.replace("_", "-", 1))
+    return names
+
+
+def perset_path(name):
+    return os.path.join(HERE, "libspx_%s.so" % name.replace("-", "_"))
+
+
+def build_perset(force=False):
+    """libspx_<set>.so for every set: csrc/perset.c with CUSPX_SET_NAME, linked
+    to libcuspx.so beside it."""
+    src = os.path.join(CSRC, "perset.c")
+    dep = max(os.path.getmtime(src), os.path.getmtime(LIB),
+              os.path.getmtime(os.path.join(ROOT, "include", "cuspx.h")))
+    for name in perset_names():
+        out = perset_path(name)
+        if not force and os.path.exists(out) and os.path.getmtime(out) >= dep:
+            continue
+        cmd = [os.environ.get("CC", "gcc"), "-O2", "-fPIC", "-shared", "-Wall", "-Wextra",
+               '-DCUSPX_SET_NAME="%s"' % name, "-I" + os.path.join(ROOT, "include"), src,
+               "-o", out + ".tmp", "-L" + HERE, "-lcuspx", "-ldl", "-Wl,-rpath,$ORIGIN"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0 or p.stderr.strip():
+            raise RuntimeError("per-set library %s failed:\n%s" % (name, p.stdout + p.stderr))
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

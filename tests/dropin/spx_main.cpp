@@ -4,6 +4,12 @@
 //
 //   g++ -DSHA256 -DSPX_128F -I include spx_main.cpp -L cuspx_amd -lcuspx
 //
+// or, with -DPLAIN_API_H, against the plain declarations and size macros
+// only (cuspx.h + cuspx_params.h, as a caller keeping its api.h and
+// params.h would), linked to the per-set library, which alone picks the set:
+//
+//   g++ -DSHA256 -DSPX_128F -DPLAIN_API_H -I include spx_main.cpp -L cuspx_amd -lspx_sha2_128f
+//
 // The DRBG is seeded with the NIST KAT entropy 00..2f; the library runs in
 // the reference's shipped DEBUG_MODE (keygen seed 0..3n-1, optrand 0xad).
 // Prints "pk", "sk", "m" and "sm" lines in hex for the test to compare
@@ -14,7 +20,12 @@
 
 #include <vector>
 
+#ifdef PLAIN_API_H
+#include "cuspx.h"
+#include "cuspx_params.h"
+#else
 #include "cuspx_api.h"
+#endif
 
 static void put_hex(const char* tag, const unsigned char* b, size_t n) {
     printf("%s ", tag);

@@ -95,6 +95,69 @@ def test_dropin_callers_build_with_gxx(tmp_path, prog, sel):
     assert p.returncode == 0, p.stderr[-3000:]
 
 
+API_NAMES = ["randombytes_init", "randombytes", "crypto_sign_secretkeybytes",
+             "crypto_sign_publickeybytes", "crypto_sign_bytes", "crypto_sign_seedbytes",
+             "crypto_sign_seed_keypair", "crypto_sign_keypair", "crypto_sign_signature",
+             "crypto_sign_verify", "crypto_sign", "crypto_sign_open",
+             "face_mdp_crypto_sign_keypair", "face_mdp_crypto_sign", "face_mdp_crypto_sign_open",
+             "face_ms_mdp_crypto_sign_keypair", "face_ms_mdp_crypto_sign",
+             "face_ms_mdp_crypto_sign_open"]
+
+
+def test_perset_libraries_export_the_api_names():
+    """One libspx_<set>.so per parameter set (the reference builds one
+    library per set), each defining every api.h / rng.h name and depending
+    on libcuspx.so."""
+    from cuspx_amd import build as B
+    names = B.perset_names()
+    assert len(names) == 12 and sorted(names) == sorted(C.params(i).name.decode()
+                                                         for i in range(12))
+    for name in names:
+        path = B.perset_path(name)
+        assert os.path.exists(path), path
+        syms = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True,
+                              text=True, check=True).stdout.split()
+        assert [s for s in API_NAMES if s not in syms] == [], name
+        dyn = subprocess.run(["readelf", "-d", path], capture_output=True, text=True,
+                             check=True).stdout
+        assert "[libcuspx.so]" in dyn, name
+
+
+def test_perset_library_alone_picks_its_set(tmp_path):
+    """A C caller with the plain declarations (no cuspx_api.h, so no
+    constructor of its own) linked to libspx_<set>.so alone sees that set's
+    sizes through the api.h getters; so does one that also links libcuspx.so
+    after it (host code only: no device call).  Listed after libcuspx.so,
+    the per-set library would be dropped by --as-needed, every name being
+    found in libcuspx.so first."""
+    src = tmp_path / "plain.c"
+    src.write_text('#include <stdio.h>\n#include "cuspx.h"\n'
+                   'int main(void){printf("%llu %llu %llu %llu\\n", crypto_sign_bytes(),'
+                   ' crypto_sign_publickeybytes(), crypto_sign_secretkeybytes(),'
+                   ' crypto_sign_seedbytes());return 0;}\n')
+    lib = os.path.join(ROOT, "cuspx_amd")
+    for name in ("shake256-192f", "sha2-128s", "sha2-256f"):
+        p = C.params(C.param_id(name))
+        want = [p.sig_bytes, p.pk_bytes, p.sk_bytes, 3 * p.n]
+        stem = "spx_" + name.replace("-", "_")
+        for order in (["-l" + stem], ["-l" + stem, "-lcuspx"]):
+            exe = tmp_path / ("plain_%s_%d" % (stem, len(order)))
+            subprocess.run(["gcc", "-Wall", "-Werror", "-I" + INC, str(src), "-o", str(exe),
+                            "-L" + lib] + order + ["-Wl,-rpath," + lib], check=True)
+            out = subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+            assert [int(x) for x in out.stdout.split()] == want, (name, order)
+
+
+def test_dropin_caller_builds_against_plain_declarations(tmp_path):
+    src = os.path.join(ROOT, "tests", "dropin", "spx_main.cpp")
+    exe = tmp_path / "plain.out"
+    p = subprocess.run(["g++", "-O1", "-std=c++11", "-Wall", "-Wextra", "-Werror",
+                        "-DSHAKE256", "-DSPX_128F", "-DPLAIN_API_H", "-I" + INC, src, "-o",
+                        str(exe), "-L" + os.path.join(ROOT, "cuspx_amd"), "-lspx_shake256_128f"],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-3000:]
+
+
 def test_api_header_selects_the_default_set(tmp_path):
     """cuspx_api.h's load-time constructor sets the library's default to the
     compiled-in set (host code only: no device call)."""

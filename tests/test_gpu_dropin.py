@@ -3,7 +3,9 @@
 * The C++ callers of tests/dropin (shaped like the reference's test/spx.cu
   and test/dp_spx.cu, built with g++ against include/cuspx_api.h only and
   linked to libcuspx.so) run and pass their own checks; their keypairs and
-  signed messages equal the CPU oracle's for the same DRBG stream.
+  signed messages equal the CPU oracle's for the same DRBG stream.  So do
+  the same single-item callers built against the plain declarations and
+  linked to the per-set library libspx_<set>.so alone.
 * The batched api.h calls in DRBG mode (the reference draws curand there,
   sign.cu:1208 / rng.cu:196; this library draws the host DRBG item by item)
   equal the oracle's keypair / signature sequence, and the multi-stream
@@ -35,8 +37,12 @@ def _run(name, *args):
 
 
 @pytest.mark.parametrize("name,prog", [("sha2-128f", "spx_sha2_128f"),
-                                       ("shake256-128f", "spx_shake256_128f")])
+                                       ("shake256-128f", "spx_shake256_128f"),
+                                       ("sha2-128f", "spx_sha2_128f_perset"),
+                                       ("shake256-128f", "spx_shake256_128f_perset")])
 def test_dropin_single_item_caller(name, prog):
+    """The *_perset callers declare the api.h names without cuspx_api.h and
+    link libspx_<set>.so alone: the per-set library picks the set."""
     out = {}
     for line in _run(prog).splitlines():
         tag, _, val = line.partition(" ")
