@@ -5,6 +5,11 @@
 //   g++ -DSHA256 -DSPX_128F -I include dp_main.cpp -L cuspx_amd -lcuspx
 //   ./dp_main NUM [OUT]
 //
+// or, with -DPLAIN_API_H, against the plain declarations and size macros
+// only, linked to the per-set library alone (as spx_main.cpp):
+//
+//   g++ -DSHA256 -DSPX_128F -DPLAIN_API_H -I include dp_main.cpp -L cuspx_amd -lspx_sha2_128f
+//
 // The DRBG is seeded with the NIST KAT entropy 00..2f and draws the NUM
 // 32-byte messages; keys and optrand follow the shipped DEBUG_MODE.  OUT (if
 // given) receives m || pk || sk || sm of the face_mdp_* calls for the test to
@@ -15,7 +20,12 @@
 
 #include <vector>
 
+#ifdef PLAIN_API_H
+#include "cuspx.h"
+#include "cuspx_params.h"
+#else
 #include "cuspx_api.h"
+#endif
 
 static int failures = 0;
 static void expect(bool ok, const char* what) {

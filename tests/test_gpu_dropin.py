@@ -60,7 +60,8 @@ def test_dropin_single_item_caller(name, prog):
 
 
 @pytest.mark.parametrize("name,prog,num", [("sha2-128f", "dp_sha2_128f", 4096),
-                                           ("shake256-128f", "dp_shake256_128f", 1100)])
+                                           ("shake256-128f", "dp_shake256_128f", 1100),
+                                           ("shake256-128f", "dp_shake256_128f_perset", 300)])
 def test_dropin_batched_caller(name, prog, num, tmp_path):
     path = str(tmp_path / "dp.bin")
     _run(prog, str(num), path)
