@@ -473,6 +473,31 @@ def test_sha2_long_messages_beyond_the_long_workgroups():
     assert np.array_equal(sig, dsig)
 
 
+@pytest.mark.parametrize("name", ["sha2-128f", "shake256-128f"])
+def test_one_mebibyte_messages(name):
+    """Messages of 1 MiB and 1 MiB + 1 bytes (the reference hashes a 1 MiB
+    patterned input, test/sha256.cu:142-182): PRF_msg and H_msg each absorb
+    16384 SHA-256 blocks (the two-wave path) or 7711 SHAKE256 rate blocks,
+    through the host-buffer and the device entry points; signatures equal
+    the oracle's and verify, and a flipped last message byte fails."""
+    pid = C.param_id(name)
+    p = O.params(pid)
+    pattern = bytes((i * 7 + 3) & 0xFF for i in range(256))
+    msgs = [pattern * 4096, pattern * 4096 + b"\x5a"]
+    rng = random.Random(1 << 20)
+    seeds = [rng.randbytes(3 * p.n) for _ in msgs]
+    opts = [rng.randbytes(p.n) for _ in msgs]
+    pk, sk = C.batch_keygen(pid, np.frombuffer(b"".join(seeds), np.uint8))
+    print("%s: sign of two 1 MiB messages" % name, flush=True)
+    sig = C.batch_sign(pid, sk, msgs, np.frombuffer(b"".join(opts), np.uint8))
+    for i, m in enumerate(msgs):
+        assert bytes(sig[i]) == O.signature_with_optrand(pid, m, bytes(sk[i]), opts[i]), i
+    dsig, dst = _device_sign_verify(pid, sk, pk, msgs, opts)
+    assert np.array_equal(dsig, sig) and list(dst) == [0, 0]
+    bad = [m[:-1] + bytes([m[-1] ^ 1]) for m in msgs]
+    assert all(s != 0 for s in C.batch_verify(pid, pk, sig, bad))
+
+
 def test_sha2_small_set_long_messages():
     """The two-wave prep path on a small set (SHA2-128s: its own digest split
     into FORS indices and a 54-bit tree index): messages at the 512-byte
