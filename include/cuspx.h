@@ -19,6 +19,15 @@
  * (sign.cu:342-349, 1950-1954); the batched face_mdp_* / face_ms_mdp_* calls
  * draw nothing (sign.cu:425-436, 2022-2027).  The library starts in that
  * mode; cuspx_set_debug_mode(0) selects DRBG-derived seeds and optrand.
+ *
+ * Threads (the reference's calls are not reentrant): the host-buffer calls
+ * (crypto_sign*, face_mdp_*, face_ms_mdp_*, spx_batch_*) may be made from
+ * several threads; on one device they take turns behind a lock.  The DRBG
+ * is one shared, locked stream, so draws made by concurrent threads
+ * interleave in no fixed order.  The device-resident calls use only the
+ * caller's buffers, workspace and stream; calls that overlap need
+ * workspaces of their own.  The default parameter set and debug mode are
+ * process-wide.
  */
 #ifndef CUSPX_H
 #define CUSPX_H
