@@ -60,6 +60,59 @@ def test_host_buffer_calls_from_four_threads():
             assert np.array_equal(a, b), name
 
 
+def test_server_submitters_and_waiters_on_four_threads():
+    """Four threads each submit 40 sign and 10 verify requests to one server
+    (three parameter sets, messages 0 B .. 5 KiB, explicit optrand) and wait
+    for them; every signature equals the host-buffer batch call's for the
+    same inputs, and every verify status is exact."""
+    names = ["sha2-128f", "sha2-128s", "shake256-256f"]
+    keys = []
+    for k, name in enumerate(names):
+        pid = C.param_id(name)
+        seeds, _, _ = _inputs(name, 2, 300 + k)
+        pk, sk = C.batch_keygen(pid, seeds)
+        keys += [(pid, bytes(sk[i]), bytes(pk[i])) for i in range(2)]
+    work = []
+    for t in range(4):
+        rng = random.Random(400 + t)
+        reqs = []
+        for _ in range(40):
+            k = rng.randrange(len(keys))
+            n = C.params(keys[k][0]).n
+            reqs.append((k, rng.randbytes(rng.choice((0, 17, 300, 2100, 5000))),
+                         rng.randbytes(n)))
+        work.append(reqs)
+    want = {}
+    for t, reqs in enumerate(work):
+        for j, (k, m, opt) in enumerate(reqs):
+            pid, sk, _ = keys[k]
+            want[(t, j)] = bytes(C.batch_sign(pid, np.frombuffer(sk, np.uint8), [m],
+                                              np.frombuffer(opt, np.uint8))[0])
+    with C.Server(hbm_budget_bytes=4 << 30, slots=4, batch_window_us=300) as srv:
+        kids = [srv.add_key(pid, sk, pk) for pid, sk, pk in keys]
+
+        def client(t):
+            reqs = work[t]
+            ts = [srv.submit_sign(kids[k], m, opt) for k, m, opt in reqs]
+            sigs = [srv.wait(x, key=kids[k]) for x, (k, _, _) in zip(ts, reqs)]
+            # verify ten of them, every other one with its last byte flipped
+            vs = []
+            for j in range(10):
+                k, m, _ = reqs[j]
+                s = sigs[j][1] if j % 2 == 0 else sigs[j][1][:-1] + bytes([sigs[j][1][-1] ^ 1])
+                vs.append(srv.submit_verify(kids[k], m, s))
+            return sigs, [srv.wait(v)[0] for v in vs]
+
+        print("four server clients", flush=True)
+        with ThreadPoolExecutor(4) as ex:
+            res = list(ex.map(client, range(4)))
+        assert srv.stats()["last_error"] == 0
+    for t, (sigs, vst) in enumerate(res):
+        for j, (st, sig) in enumerate(sigs):
+            assert st == 0 and sig == want[(t, j)], (t, j)
+        assert vst == [0 if j % 2 == 0 else -1 for j in range(10)], t
+
+
 def _device_job(seed, num=64):
     pid = C.param_id("sha2-128f")
     p = C.params(pid)
