@@ -511,51 +511,59 @@ struct Sha2 {
     // mlen are never read beyond the word holding the last byte.  Blocks made
     // only of whole message words are loaded as blocks (stream_full, or
     // stream_msg_long's producing wave); the head (prefix) and the tail (last
-    // partial word, padding, length) go word by word.
+    // partial word, padding, length) go through stream_range.
     //
-    // stream_range runs the words [kk, kk_end) of that stream word by word
-    // (kk a multiple of 16; kk_end clipped to the padded end).
+    // stream_range runs the blocks of words [kk, kk_end) of that stream (kk a
+    // multiple of 16; kk_end clipped to the padded end).  Each block's 16
+    // words are formed in one unrolled pass, so the loads of its message
+    // words are issued together (word by word, each load stood on the serial
+    // path: a short message's two passes cost one load latency per word).
     template <int NPRE>
     __device__ static void stream_range(uint32_t st[8], const uint32_t pre[NPRE],
                                         const uint8_t* m, uint32_t mlen, uint32_t before,
                                         uint32_t kk, uint32_t kk_end) {
+        static_assert(NPRE <= 32, "the prefix ends within the first two blocks");
         const uint64_t bits = ((uint64_t)before + 4u * NPRE + mlen) * 8u;
         const uint32_t ndata = NPRE + (mlen + 4) / 4;  // words holding data or 0x80
         const uint32_t total = ((ndata + 2 + 15) / 16) * 16;
         if (kk_end > total) kk_end = total;
         const gbl_u32* mw = (gbl_u32*)m;  // address-space cast
-        uint32_t sl[8], b[16];  // the chaining value stays in registers
+        uint32_t sl[8];  // the chaining value stays in registers
 #pragma unroll
         for (int i = 0; i < 8; i++) sl[i] = st[i];
-#pragma unroll
-        for (int i = 0; i < 16; i++) b[i] = 0;
 #pragma unroll 1
-        while (kk < kk_end) {
-            uint32_t x;
-            if (kk < (uint32_t)NPRE) {
-                x = pre[0];
+        for (; kk < kk_end; kk += 16) {
+            uint32_t b[16];
 #pragma unroll
-                for (int j = 1; j < NPRE; j++) x = (kk == (uint32_t)j) ? pre[j] : x;
-            } else if (kk < ndata) {
-                const uint32_t i = kk - NPRE;
-                const int valid = (int)mlen - 4 * (int)i;
-                const uint32_t raw = valid > 0 ? bswap32(mw[i]) : 0u;
-                if (valid >= 4) {
-                    x = raw;
+            for (int j = 0; j < 16; j++) {
+                const uint32_t w = kk + (uint32_t)j;
+                uint32_t x;
+                if (w < (uint32_t)NPRE) {
+                    // the prefix lies in block 0 (kk = 0) or block 1 (kk = 16)
+                    uint32_t pv = j < NPRE ? pre[j < NPRE ? j : 0] : 0u;
+                    if (16 + j < NPRE) pv = kk ? pre[16 + j < NPRE ? 16 + j : 0] : pv;
+                    x = pv;
+                } else if (w < ndata) {
+                    const uint32_t i = w - NPRE;
+                    const int valid = (int)mlen - 4 * (int)i;  // 0 .. mlen
+                    const uint32_t raw = valid > 0 ? bswap32(mw[i]) : 0u;
+                    if (valid >= 4) {
+                        x = raw;
+                    } else {
+                        const uint32_t keep =
+                            valid <= 0 ? 0u : (0xFFFFFFFFu << (8 * (4 - valid)));
+                        x = (raw & keep) | (0x80000000u >> (8 * (valid < 0 ? 0 : valid)));
+                    }
+                } else if (w == total - 2) {
+                    x = (uint32_t)(bits >> 32);
+                } else if (w == total - 1) {
+                    x = (uint32_t)bits;
                 } else {
-                    const uint32_t keep = valid <= 0 ? 0u : (0xFFFFFFFFu << (8 * (4 - valid)));
-                    x = (raw & keep) | (0x80000000u >> (8 * (valid < 0 ? 0 : valid)));
+                    x = 0;
                 }
-            } else if (kk == total - 2) {
-                x = (uint32_t)(bits >> 32);
-            } else if (kk == total - 1) {
-                x = (uint32_t)bits;
-            } else {
-                x = 0;
+                b[j] = x;
             }
-            put16(b, (int)(kk & 15), x);
-            if ((kk & 15) == 15) sha256_compress_dup(sl, b);
-            kk++;
+            sha256_compress_dup(sl, b);
         }
 #pragma unroll
         for (int i = 0; i < 8; i++) st[i] = sl[i];
