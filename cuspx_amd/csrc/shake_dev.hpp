@@ -478,23 +478,28 @@ struct Shake {
 #pragma unroll
             for (int l = 0; l < 17; l++) s[l] ^= (qq >> 1) == l ? c : 0u;
         };
-        // word by word up to the first block boundary, then the blocks made
-        // only of message words (block b + 1 loads while block b permutes),
-        // then the words left over
-        auto words = [&](uint32_t stop) {
-#pragma unroll 1
-            while (i < stop) {
-                put(q, mw[i]);
-                q++;
-                i++;
-                if (q == 34) {
-                    keccak_f_il_pair(s);
-                    q = 0;
-                }
+        // the message words up to the first block boundary (stream positions
+        // NPRE .. 33, fixed at compile time), then the blocks made only of
+        // message words (block b + 1 loads while block b permutes), then the
+        // words left over (positions 0 .. 32 of the last block); the head's
+        // and the tail's words are each loaded together (one word per step
+        // had put a load on the serial path for every word)
+        const gbl_u32* mg = (gbl_u32*)mw;  // address-space cast
+        constexpr int HEAD = NPRE == 0 ? 0 : 34 - NPRE;
+        if constexpr (HEAD > 0) {
+            const uint32_t nh = (uint32_t)HEAD < full ? (uint32_t)HEAD : full;
+            uint32_t x[HEAD];
+#pragma unroll
+            for (int k = 0; k < HEAD; k++) x[k] = (uint32_t)k < nh ? mg[k] : 0u;
+#pragma unroll
+            for (int k = 0; k < HEAD; k++) s[(NPRE + k) >> 1] ^= half(x[k], NPRE + k);
+            i = nh;
+            q = NPRE + (int)nh;
+            if (q == 34) {
+                keccak_f_il_pair(s);
+                q = 0;
             }
-        };
-        const uint32_t head = q == 0 ? 0u : 34u - (uint32_t)q;
-        words(head < full ? head : full);
+        }
         const uint32_t nfb = q == 0 ? (full - i) / 34 : 0u;
         if (nfb) {
             const gbl_u32* g = (gbl_u32*)mw + i;  // address-space cast
@@ -514,7 +519,16 @@ struct Shake {
             }
             i += 34 * nfb;
         }
-        words(full);
+        if (q == 0) {  // else the message ended inside the first block: i == full
+            const uint32_t r = full - i;  // < 34
+            uint32_t x[34];
+#pragma unroll
+            for (int k = 0; k < 34; k++) x[k] = (uint32_t)k < r ? mg[i + k] : 0u;
+#pragma unroll
+            for (int k = 0; k < 34; k++) s[k >> 1] ^= half(x[k], k);
+            q = (int)r;
+            i = full;
+        }
         uint32_t last = rem ? (mw[full] & (0xFFFFFFFFu >> (8 * (4 - rem)))) : 0u;
         last |= 0x1Fu << (8 * rem);
         put(q, last);
