@@ -500,6 +500,8 @@ def main():
               "between GPUs, so one MI355X's throughput is the figure. Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0 or args.batch < 1:
+        ap.error("--steps and --batch must be at least 1 and --warmup at least 0")
 
     import numpy as np
     import cuspx_amd as C
@@ -564,7 +566,8 @@ def main():
         sign()
         verify()
     stream.sync()
-    assert all_valid(), "batched verify rejected a batched signature"
+    if args.warmup > 0:  # with no warm-up, the statuses still hold the 0xFF fill
+        assert all_valid(), "batched verify rejected a batched signature"
 
     C.profile_reset()
     C.profile_enable(True)
