@@ -404,6 +404,13 @@ def mixed_phase(C, n, reps=3, seed=3):
     items = [(C.param_id(name), sk, m, opt) for (name, _, m, opt), sk in zip(work, sks)]
     mb = C.MixedSignBatch(items)
     st = C.Stream()
+    # which path the batch takes: the graph is captured from the same calls
+    C.profile_reset()
+    C.profile_enable(True)
+    mb.launch(st, eager=True)
+    st.sync()
+    fused = "k_sign_family" in C.profile_read()
+    C.profile_enable(False)
     mb.launch(st)
     st.sync()
     ts = []
@@ -425,9 +432,11 @@ def mixed_phase(C, n, reps=3, seed=3):
     return {"sets": ["shake256-128f", "shake256-192f"], "requests": len(items),
             "message_bytes": "log-uniform 1..4096", "ms_per_batch": round(ms, 3),
             "signs_per_s": round(len(items) / ms * 1e3, 1), "spot_check_invalid": bad,
-            "launch": "one hipGraph per replay: both sets' prep kernels, one persistent "
-                      "family kernel over every set's tiles, then T_k and the WOTS gather "
-                      "per set"}
+            "launch": ("one hipGraph per replay: both sets' prep kernels, one persistent "
+                       "family kernel over every set's tiles, then T_k and the WOTS gather "
+                       "per set") if fused else
+                      ("one hipGraph per replay: each set's pipeline on its own forked "
+                       "stream")}
 
 
 def graph_128s_phase(C, sizes=(1, 64, 1024), reps=3, seed=4):

@@ -655,8 +655,9 @@ class MixedSignBatch:
     """A resident mixed-parameter-set signing batch: requests are grouped by
     set, each group gets device buffers and a workspace, and the whole batch
     is one hipGraph (one launch per replay).  Distinct fast sets of one hash
-    family run as one persistent kernel; other mixes run each set's pipeline
-    on its own forked stream.  launch(eager=True) launches the same batch
+    family, each large enough for one or two lanes per WOTS leaf, run as one
+    persistent kernel; other mixes run each set's pipeline on its own forked
+    stream.  launch(eager=True) launches the same batch
     without the graph.
 
     items: list of (pid, sk bytes, message bytes, optrand bytes)."""

@@ -1170,8 +1170,8 @@ static int check_groups(const cuspx_sign_group* groups, int ngroups) {
 
 // A mixed batch whose non-empty groups are distinct fast sets of one hash
 // family runs as one persistent kernel (FamilyOps); returns kNotFused if the
-// groups do not form such a batch or a group would not run at one lane per
-// WOTS leaf (nothing launched then).
+// groups do not form such a batch or the split rule would run a group at
+// more than two lanes per WOTS leaf (nothing launched then).
 static int sign_family(const cuspx_sign_group* groups, int ngroups, hipStream_t s) {
     const FamilyOps* fams[2] = {cuspx_family_sha2_f(), cuspx_family_shake256_f()};
     for (const FamilyOps* f : fams) {

@@ -200,8 +200,8 @@ int cuspx_graph_verify(int pid, uint32_t num, const uint8_t *pk, uint32_t pk_str
 /* A mixed batch: `groups` single-set sign batches (device-resident, as in
  * cuspx_sign_device; each with its own workspace).  When the non-empty
  * groups are distinct fast sets of one hash family (sha2-128f/192f/256f or
- * shake256-128f/192f/256f) and each runs at one lane per WOTS leaf, they
- * are signed by ONE persistent kernel whose tile queue spans every set
+ * shake256-128f/192f/256f) and each is large enough to run at one or two
+ * lanes per WOTS leaf, they are signed by ONE persistent kernel whose tile queue spans every set
  * (after each set's prep kernel; T_k and the WOTS gather per set after it).
  * Other mixes run each group's own pipeline on a forked stream.  Not in the
  * reference (its batches are single-set, sign.cu:5098). */

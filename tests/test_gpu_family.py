@@ -3,9 +3,9 @@ kernel (k_sign_family): every signature equals the CPU oracle's, launched
 directly and as a graph replayed twice; mixes outside one family keep the
 per-set pipelines and are checked the same way.
 
-The fused path applies when every set's batch runs at one lane per WOTS
-leaf; small test batches force that split (cuspx_set_split) so the fused
-kernel itself is what runs, with batches whose last workgroups are partly
+The fused path applies when every set's batch runs at one or two lanes
+per WOTS leaf; small test batches force such a split (cuspx_set_split) so
+the fused kernel itself is what runs, with batches whose last workgroups are partly
 empty and with empty sets."""
 import random
 
@@ -66,6 +66,34 @@ def test_family_kernel_matches_oracle(family, counts):
             st.sync()
             assert mb.signatures() == eager
     finally:
+        for pid in pids:
+            C.set_split(pid, C.OP_SIGN, 0)
+
+
+def test_family_kernel_taken_up_to_two_lanes_per_leaf():
+    """The split rule picks the path: at two lanes per WOTS leaf the batch
+    still runs as k_sign_family, at four it runs the per-set pipelines; the
+    bytes are the oracle's either way."""
+    names = FAMILIES["shake256"]
+    pids = [C.param_id(n) for n in names]
+    items = _items(names, (3, 2, 0), 41)
+    st = C.Stream()
+    try:
+        for split, fused in ((2, True), (4, False)):
+            for pid in pids:
+                C.set_split(pid, C.OP_SIGN, split)
+            print("shake256 family batch at %d lanes per leaf" % split, flush=True)
+            mb = C.MixedSignBatch(items)
+            C.profile_reset()
+            C.profile_enable(True)
+            mb.launch(st, eager=True)
+            st.sync()
+            prof = C.profile_read()
+            C.profile_enable(False)
+            assert ("k_sign_family" in prof) == fused, (split, sorted(prof))
+            _check(items, mb.signatures())
+    finally:
+        C.profile_enable(False)
         for pid in pids:
             C.set_split(pid, C.OP_SIGN, 0)
 
