@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/spx_oracle.c).
 
-TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP engine in
-``cuspx_amd``: only ``tests/``, ``__graft_entry__.smoke()`` and the
+TEST INFRASTRUCTURE ONLY.  The oracle is what the HIP engine in
+``cuspx_amd`` is tested against: only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import this module.  The product
 path never loads it.
 
