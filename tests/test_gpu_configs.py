@@ -142,9 +142,24 @@ def test_config3_mixed_shake_8192_one_graph():
     print("config3: mixed graph of %d" % len(batch), flush=True)
     mb = C.MixedSignBatch(batch)
     st = C.Stream()
+    # the batch runs as ONE persistent kernel over both sets' tiles (the
+    # graph is captured from the same calls as this eager launch)
+    C.profile_reset()
+    C.profile_enable(True)
+    try:
+        mb.launch(st, eager=True)
+        st.sync()
+        prof = C.profile_read()
+    finally:
+        C.profile_enable(False)
+    assert "k_sign_family" in prof, sorted(prof)
+    eager = mb.signatures()
+    for d in mb.bufs:
+        d["sig"].fill(0)
     mb.launch(st)
     st.sync()
     sigs = mb.signatures()
+    assert sigs == eager
     _compare("config3", sigs, _digests(g["sig"]))
     # every signature verifies (one batch per set)
     for name in g["sets"]:
